@@ -1,0 +1,158 @@
+"""Benchmark: aggregate frames/s of the batched Atari 2600 frame step on one GPU.
+
+Workload (one "step" = one frame of every env): the seeded 8 KB F8
+bankswitched WSYNC ROM (bank switch every 16 scanlines), 65,536 envs,
+per-env SplitMix64 action streams, actions resident in HBM before timing.
+`value` is the plain per-frame path (fa_step_device, k = 1); the batched path
+(step_multi, K = 16 frames per launch) is reported beside it.  The CPU
+baseline is the oracle (the C restatement of the reference emulator) on a
+bounded sample of the same workload on the host's cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "fast-atari-rs_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes per env-frame of the per-frame step: read state 222 +
+# action 1, write state 222 + obs 128.
+BYTES_PER_ENV_FRAME = 222 + 1 + 222 + 128
+N_ENVS = 65536
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(rom, n_sample, frames, seed, threads):
+    from oracle import oracle
+    from fastatari import workloads as W
+    b = oracle.Batch(rom, n_sample, nthreads=threads)
+    b.reset()
+    acts = W.ActionStream(seed, n_sample).frames(frames)
+    secs = b.bench(acts, threads)
+    return n_sample * frames / secs, secs
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=320)
+    ap.add_argument("--warmup", type=int, default=16)
+    ap.add_argument("--k", type=int, default=16, help="frames per launch of the batched path")
+    ap.add_argument("--envs", type=int, default=N_ENVS)
+    ap.add_argument("--sorted", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(envs are independent; sharding is a per-device split of the env range).", file=sys.stderr)
+        sys.exit(2)
+
+    import fastatari
+    from fastatari import workloads as W
+
+    seed = W.config_seed(2)
+    rom = W.rom_for(2)
+    n = args.envs
+    steps, warm, k = args.steps, args.warmup, args.k
+    steps = max(k, (steps // k) * k)
+
+    log(f"bench: F8 WSYNC ROM, {n} envs, warmup {warm}, timed {steps} frames (plain) + {steps} frames (K={k})")
+    acts = W.ActionStream(seed, n).frames(warm + 2 * steps)            # [frames, n]
+    d_act = fastatari.DeviceBuffer(acts.nbytes)
+    d_act.upload(acts)
+    d_obs = fastatari.DeviceBuffer(n * 128)
+
+    gpu = fastatari.BatchAtariGpu(rom, n)
+    gpu.set_sorted(args.sorted)
+    gpu.reset()
+    gpu.set_timing(True)
+
+    # ---- plain per-frame path (the drop-in for atari_frame_kernel)
+    for f in range(warm):
+        gpu.step_device(d_act.at(f * n), 1, d_obs.ptr)
+    gpu.sync()
+    gpu.kernel_time(reset=True)
+    t0 = time.perf_counter()
+    for f in range(steps):
+        gpu.step_device(d_act.at((warm + f) * n), 1, d_obs.ptr)
+    gpu.sync()
+    t1 = time.perf_counter()
+    plain_ms_total, plain_launches = gpu.kernel_time(reset=True)
+    plain_wall = t1 - t0
+    plain_fps = n * steps / plain_wall
+    kernel_ms = plain_ms_total / max(plain_launches, 1)
+
+    # ---- batched path: K frames per launch, state kept on-chip across frames
+    t0 = time.perf_counter()
+    for L in range(steps // k):
+        gpu.step_device(d_act.at((warm + steps + L * k) * n), k, d_obs.ptr)
+    gpu.sync()
+    t1 = time.perf_counter()
+    multi_ms_total, multi_launches = gpu.kernel_time(reset=True)
+    multi_wall = t1 - t0
+    multi_fps = n * steps / multi_wall
+
+    counters = gpu.counters()
+    instr_total = int(counters["instructions"].sum())
+    frames_total = n * (1 + warm + 2 * steps)
+    status_bad = int((counters["status"] != 0).sum())
+
+    # ---- API-level rate (host actions in, host obs out each frame, as the reference measures)
+    api_frames = 16
+    t0 = time.perf_counter()
+    for f in range(api_frames):
+        gpu.step(acts[f])
+    api_fps = n * api_frames / (time.perf_counter() - t0)
+    gpu.kernel_time(reset=True)
+
+    achieved_gbs = BYTES_PER_ENV_FRAME * n / (kernel_ms * 1e-3) / 1e9
+    out = {
+        "metric": "aggregate frames/sec at 64K envs (plain vs batched); p50/p99 served step latency",
+        "value": round(plain_fps, 1),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warm,
+        "ms_per_step": round(plain_wall / steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded 8 KB F8 ROM, SplitMix64 per-env actions)",
+        "config": {"workload": "F8 8KB WSYNC ROM (bank switch every 16 lines), per-frame step",
+                   "envs": n, "frames": steps, "sorted": bool(args.sorted), "block": gpu.block_size()},
+        "batched": {"k": k, "fps": round(multi_fps, 1), "ms_per_launch": round(multi_wall / (steps // k) * 1e3, 4),
+                    "kernel_ms_per_launch": round(multi_ms_total / max(multi_launches, 1), 4)},
+        "api_level_fps": round(api_fps, 1),
+        "kernel_ms_per_frame_step": round(kernel_ms, 4),
+        "emulated_instr_per_s": round(instr_total / frames_total * plain_fps, 1),
+        "instructions_per_frame": round(instr_total / frames_total, 1),
+        "env_errors": status_bad,
+        "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 6), "traffic": None},
+    }
+    if not args.no_cpu:
+        sample_n, sample_frames = 8192, 48
+        cpu_fps, secs = cpu_baseline(rom, sample_n, sample_frames, seed, args.cpu_threads)
+        out["cpu_baseline"] = {"value": round(cpu_fps, 1), "unit": "frames/s", "cores": args.cpu_threads,
+                               "kind": "port",
+                               "sample": f"{sample_n} envs x {sample_frames} frames of the same ROM/actions "
+                                         f"({secs:.1f} s)"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,11 @@
+"""fastatari -- MI355X batched Atari 2600 frame step.
+
+`BatchAtariGpu` mirrors the reference's Rust API (src/cuda_env.rs:99-191):
+new / set_sorted / reset / step / step_multi / download_states / num_envs,
+over the C ABI in include/fastatari.h (libfastatari_hip.so, HIP kernels for
+gfx950).  There is no CPU fallback: constructing a BatchAtariGpu without the
+built library or without a GPU raises.
+"""
+from .env import (STATE_DTYPE, BatchAtariGpu, FastAtariError, DeviceBuffer, lib_path, load_library)
+
+__all__ = ["BatchAtariGpu", "FastAtariError", "DeviceBuffer", "STATE_DTYPE", "lib_path", "load_library"]

@@ -1,0 +1,152 @@
+/*
+ * fastatari.h -- C ABI of the MI355X batched Atari 2600 frame step.
+ *
+ * Drop-in boundary for the reference's GPU path (RPP1011/fast-atari-rs):
+ * every entry point below names the reference interface it replaces.  The
+ * handle owns all device memory; host buffers are plain pointers and sizes.
+ * Every function returns FA_OK (0) or a negative FA_E* code; fa_last_error()
+ * gives a message for the calling thread's last failure.  A handle is not
+ * thread-safe (as the reference's `&mut self` methods), calls on one handle
+ * are serialised on the handle's own HIP stream.
+ */
+#ifndef FASTATARI_H
+#define FASTATARI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define FA_OK 0
+#define FA_EINVAL -1     /* bad argument (wrong length, NULL pointer, bad ROM size) */
+#define FA_EHIP -2       /* HIP runtime error */
+#define FA_ESTATE -3     /* an uploaded state is not one the emulator can reach */
+#define FA_ENOMEM -4     /* device allocation failed */
+
+/* ---- per-env emulation status (fa_download_counters) ---- */
+#define FA_ENV_OK 0
+#define FA_ENV_ILLEGAL_OPCODE 1   /* reference panics here (src/cpu.rs:1139-1141) */
+#define FA_ENV_VSYNC_HELD 2       /* VSYNC held > 1,000,000 cycles (reference loops forever) */
+
+/*
+ * AtariStateGpu -- the reference's packed per-env state, byte for byte
+ * (src/cuda_env.rs:12-38, cuda/src/state_layout.cuh:16-88): 222 bytes, align 2.
+ */
+typedef struct AtariStateGpu {
+    uint8_t a, x, y, sp;
+    uint16_t pc;
+    uint8_t status, _pad0;
+    uint8_t ram[128];
+    uint8_t port_a_output, port_a_ddr, port_a_input;
+    uint8_t port_b_output, port_b_ddr, port_b_input;
+    uint16_t timer_value, timer_prescaler, timer_prescaler_select;
+    uint8_t timer_underflow, _pad1, _pad2, _pad3;
+    uint16_t tia_clock, tia_scanline;
+    uint8_t tia_wsync, tia_frame_complete, tia_vsync, tia_vblank;
+    uint8_t cxm0p, cxm1p, cxp0fb, cxp1fb, cxm0fb, cxm1fb, cxblpf, cxppmm;
+    uint8_t inpt4, inpt5, paddle0, paddle1;
+    uint16_t paddle_counter;
+    uint8_t paddle_dumped, _pad4;
+    uint16_t pos_p0, pos_p1, pos_m0, pos_m1, pos_bl;
+    uint8_t hmp0, hmp1, hmm0, hmm1, hmbl, _pad5;
+    uint8_t grp0, grp1, grp0_old, grp1_old;
+    uint8_t enam0, enam1, enabl, enabl_old;
+    uint8_t vdelp0, vdelp1, vdelbl;
+    uint8_t resmp0, resmp1, nusiz0, nusiz1;
+    uint8_t colup0, colup1, colupf, colubk;
+    uint8_t ctrlpf, refp0, refp1;
+    uint8_t pf0, pf1, pf2;
+    uint8_t bank, scheme;
+    uint8_t _pad_end[2];
+} AtariStateGpu;
+
+#define FA_STATE_BYTES 222
+#define FA_OBS_BYTES 128
+
+typedef struct fa_env fa_env;
+
+/* BatchAtariGpu::new(rom, n) -- src/cuda_env.rs:112-126.
+ * rom_len: 1..4096 (Fixed, mirrored), 8192 (F8), 16384 (F6), >= 32768 (F4;
+ * the first 32 KB are used) -- the sizes BankScheme::detect + load_bank accept
+ * without panicking (src/atari.rs:97-104, 433-449). */
+int fa_create(const uint8_t *rom, size_t rom_len, size_t n, fa_env **out);
+
+/* Drop: releases device memory (src/cuda_env.rs:99-109 owns the same buffers). */
+void fa_destroy(fa_env *env);
+
+/* BatchAtariGpu::set_sorted -- src/cuda_env.rs:129.  Non-zero enables the
+ * wave64 opcode-grouping interpreter. */
+int fa_set_sorted(fa_env *env, int sorted);
+
+/* BatchAtariGpu::reset -- src/cuda_env.rs:131-142: every env is powered on
+ * (HeadlessAtari::new) and runs one frame without an action.  obs_out
+ * (n * 128 bytes, may be NULL) receives each env's RAM. */
+int fa_reset(fa_env *env, uint8_t *obs_out);
+
+/* BatchAtariGpu::step -- src/cuda_env.rs:144-161.  actions: n bytes (action
+ * index 0..17, others = NOOP); obs_out: n * 128 bytes.  n_actions != n is
+ * FA_EINVAL (the reference asserts). */
+int fa_step(fa_env *env, const uint8_t *actions, size_t n_actions, uint8_t *obs_out);
+
+/* BatchAtariGpu::step_multi -- src/cuda_env.rs:166-184.  actions: k * n
+ * bytes laid out [f * n + i]; obs_out: RAM after the last frame, n * 128. */
+int fa_step_multi(fa_env *env, const uint8_t *actions, size_t n_actions, uint32_t k, uint8_t *obs_out);
+
+/* BatchAtariGpu::download_states -- src/cuda_env.rs:186-188. */
+int fa_download_states(fa_env *env, AtariStateGpu *out, size_t n);
+
+/* BatchAtariGpu::num_envs -- src/cuda_env.rs:190. */
+size_t fa_num_envs(const fa_env *env);
+
+/* Message for the calling thread's last non-OK return. */
+const char *fa_last_error(void);
+
+/* ---- extensions (no reference counterpart; they ride alongside) ---- */
+
+/* Replace every env's state (checkpoint restore, seeding).  States must be
+ * reachable ones: clock % 3 == 0 and < 228, select in {1, 8, 64, 1024},
+ * prescaler <= select, bank valid for the ROM, scheme equal to the ROM's. */
+int fa_upload_states(fa_env *env, const AtariStateGpu *in, size_t n);
+
+/* Per-env counters: cumulative Cpu.cycles (src/cpu.rs:104), cycles of the
+ * last frame (HeadlessAtari::run_frame's return value, src/atari.rs:584-598),
+ * cumulative executed instructions, and FA_ENV_* status.  Any pointer may be
+ * NULL. */
+int fa_download_counters(fa_env *env, uint64_t *cpu_cycles, uint64_t *frame_cycles,
+                         uint64_t *instructions, uint32_t *status);
+int fa_upload_cpu_cycles(fa_env *env, const uint64_t *cpu_cycles, size_t n);
+
+/* Hold each env's action for fs[i] frames (set_action + fs[i] x run_frame);
+ * obs after each env's last frame. */
+int fa_step_frameskip(fa_env *env, const uint8_t *actions, const uint8_t *fs, size_t n, uint8_t *obs_out);
+
+/* Device-resident variants: d_actions (k * n bytes, or n with d_fs) and
+ * d_obs (n * 128, may be NULL) are device pointers; nothing is copied and the
+ * call returns after enqueueing on the handle's stream. */
+int fa_step_device(fa_env *env, const uint8_t *d_actions, uint32_t k, uint8_t *d_obs);
+int fa_step_frameskip_device(fa_env *env, const uint8_t *d_actions, const uint8_t *d_fs, uint8_t *d_obs);
+int fa_sync(fa_env *env);
+
+/* Device memory helpers for callers without a GPU framework. */
+int fa_device_alloc(size_t bytes, void **d_ptr);
+int fa_device_free(void *d_ptr);
+int fa_memcpy_h2d(void *d_dst, const void *h_src, size_t bytes);
+int fa_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes);
+
+/* Kernel timing: when enabled, every frame-step launch is bracketed by HIP
+ * events on the handle's stream; fa_kernel_time returns the summed kernel
+ * milliseconds and the launch count since the last reset. */
+int fa_set_timing(fa_env *env, int enabled);
+int fa_kernel_time(fa_env *env, double *total_ms, uint64_t *launches, int reset);
+
+/* Launch shape: workgroup size (multiple of 64, <= 1024; 0 = automatic). */
+int fa_set_block_size(fa_env *env, uint32_t block);
+int fa_get_block_size(const fa_env *env, uint32_t *block);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,220 @@
+"""GPU parity: the HIP frame step against the oracle, bit-exact on full packed
+state (222 B), cumulative CPU cycles, last-frame cycles, instruction counts
+and error status -- every frame for the per-frame path, at every launch
+boundary for step_multi (test shapes of the reference's tests/test_cuda.rs,
+tests/test_cuda_persistent.rs and tests/test_cuda_sorted.rs, on the seeded
+synthetic ROMs)."""
+import numpy as np
+import pytest
+
+from gpu_util import assert_same
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def W():
+    from fastatari import workloads
+    return workloads
+
+
+@pytest.fixture(scope="module")
+def fa():
+    import fastatari
+    fastatari.load_library()
+    return fastatari
+
+
+def _pair(fa, orc, rom, n, sorted_=False, nthreads=16):
+    gpu = fa.BatchAtariGpu(rom, n)
+    gpu.set_sorted(sorted_)
+    ref = orc.Batch(rom, n, nthreads=nthreads)
+    obs = gpu.reset()
+    ref.reset()
+    assert np.array_equal(obs, ref.obs())
+    return gpu, ref
+
+
+@pytest.mark.parametrize("which", ["c1", "c2", "c3", "mix0", "mix3"])
+def test_reset_matches_oracle(fa, orc, W, which):
+    rom = {"c1": lambda: W.rom_for(1), "c2": lambda: W.rom_for(2), "c3": lambda: W.rom_for(3),
+           "mix0": lambda: W.rom_for(4, 0), "mix3": lambda: W.rom_for(4, 3)}[which]()
+    gpu, ref = _pair(fa, orc, rom, 130)
+    assert_same(gpu, ref, f"reset {which}")
+
+
+def test_config1_every_frame(fa, orc, W):
+    """Config 1: 1,000 envs x 500 frames, per-frame step, everything compared
+    every frame (tests/test_cuda.rs:75-116 shape)."""
+    rom = W.rom_for(1)
+    n, frames = 1000, 500
+    gpu, ref = _pair(fa, orc, rom, n)
+    acts = W.ActionStream(W.config_seed(1), n)
+    for f in range(frames):
+        a = acts.next()
+        obs = gpu.step(a).reshape(n, 128)
+        ref.run(a[None, :])
+        assert np.array_equal(obs, ref.obs()), f"obs frame {f}"
+        assert_same(gpu, ref, f"frame {f}")
+
+
+@pytest.mark.parametrize("sorted_", [False, True])
+def test_config2_step_multi_k16(fa, orc, W, sorted_):
+    """Config 2 (F8) with step_multi K=16, compared at every launch boundary,
+    and the per-frame path over the same frames."""
+    rom = W.rom_for(2)
+    n, launches, k = 2048, 6, 16
+    gpu, ref = _pair(fa, orc, rom, n, sorted_)
+    acts = W.ActionStream(W.config_seed(2), n)
+    for L in range(launches):
+        a = acts.frames(k)
+        obs = gpu.step_multi(a, k).reshape(n, 128)
+        ref.run(a)
+        assert np.array_equal(obs, ref.obs()), f"obs launch {L}"
+        assert_same(gpu, ref, f"launch {L}")
+
+
+def test_config2_per_frame(fa, orc, W):
+    rom = W.rom_for(2)
+    n, frames = 1024, 64
+    gpu, ref = _pair(fa, orc, rom, n)
+    acts = W.ActionStream(W.config_seed(2), n)
+    for f in range(frames):
+        a = acts.next()
+        gpu.step(a)
+        ref.run(a[None, :])
+        assert_same(gpu, ref, f"frame {f}")
+
+
+@pytest.mark.parametrize("sorted_", [False, True])
+def test_config3_branchy(fa, orc, W, sorted_):
+    """Config 3 (F6, no WSYNC, all 151 opcodes, data-dependent branches),
+    sorted and unsorted (tests/test_cuda_sorted.rs shape)."""
+    rom = W.rom_for(3)
+    n, frames = 1024, 12
+    gpu, ref = _pair(fa, orc, rom, n, sorted_)
+    acts = W.ActionStream(W.config_seed(3), n)
+    for f in range(frames):
+        a = acts.next()
+        gpu.step(a)
+        ref.run(a[None, :])
+        assert_same(gpu, ref, f"frame {f}")
+
+
+@pytest.mark.parametrize("member", [0, 1, 2, 3])
+def test_mix_frameskip(fa, orc, W, member):
+    """Each ROM of the heterogeneous mix with per-env frameskip 1-4."""
+    rom = W.rom_for(4, member)
+    n = 512
+    gpu, ref = _pair(fa, orc, rom, n)
+    acts = W.ActionStream(W.config_seed(4) + member, n)
+    fs = W.frameskips(W.config_seed(4) + member, n)
+    for step in range(4):
+        a = acts.next()
+        gpu.step_frameskip(a, fs)
+        ref.run_frameskip(a, fs)
+        assert_same(gpu, ref, f"member {member} step {step}")
+
+
+def test_upload_states_roundtrip(fa, orc, W):
+    """Upload oracle states taken mid-run (RAM seeded per env), continue both."""
+    rom = W.rom_for(3)
+    n = 256
+    ref = orc.Batch(rom, n, nthreads=8)
+    ref.reset()
+    acts = W.ActionStream(99, n)
+    ref.run(acts.frames(3))
+    st = ref.states()
+    rng = np.random.default_rng(5)
+    st[:, 8:8 + 0x5F] = rng.integers(0, 256, (n, 0x5F), dtype=np.uint8)   # seed data RAM per env
+    ref.set_states(st)
+    gpu = fa.BatchAtariGpu(rom, n)
+    gpu.upload_states(st)
+    cyc = ref.counters()[:, 0].copy()
+    gpu.upload_cpu_cycles(cyc)
+    # instruction counters restart from zero on the GPU: compare deltas
+    ins0 = ref.counters()[:, 2].copy()
+    for f in range(5):
+        a = acts.next()
+        gpu.step(a)
+        ref.run(a[None, :])
+        g = gpu.download_states_bytes()
+        assert np.array_equal(g, ref.states()), f"frame {f}"
+        c = gpu.counters()
+        rc = ref.counters()
+        assert np.array_equal(c["cpu_cycles"], rc[:, 0])
+        assert np.array_equal(c["instructions"], rc[:, 2] - ins0)
+
+
+def _rom_with(prog, at=0xF000):
+    rom = bytearray([0xEA] * 4096)
+    rom[at - 0xF000: at - 0xF000 + len(prog)] = bytes(prog)
+    rom[0xFFC], rom[0xFFD] = 0x00, 0xF0
+    return bytes(rom)
+
+
+def test_illegal_opcode_sets_status(fa, orc):
+    """Reference panics on an illegal opcode (src/cpu.rs:1139-1141): both
+    sides latch status 1 with PC past the opcode and freeze the env."""
+    rom = _rom_with([0x78, 0xA2, 0x05, 0xCA, 0xD0, 0xFD, 0x02])   # SEI; LDX #5; loop DEX/BNE; .byte $02
+    gpu, ref = _pair(fa, orc, rom, 64)
+    assert_same(gpu, ref, "after reset")
+    assert int(gpu.counters()["status"][0]) == 1
+    a = np.zeros(64, np.uint8)
+    gpu.step(a)
+    ref.run(a[None, :])
+    assert_same(gpu, ref, "frozen")
+
+
+def test_frame_cap_without_vsync(fa, orc):
+    """A ROM that never raises VSYNC ends each frame at the 100,000-cycle cap
+    (src/atari.rs:592-595)."""
+    rom = _rom_with([0xE6, 0x80, 0x4C, 0x00, 0xF0])              # INC $80; JMP $F000
+    gpu, ref = _pair(fa, orc, rom, 64)
+    for f in range(3):
+        a = np.full(64, f, np.uint8)
+        gpu.step(a)
+        ref.run(a[None, :])
+        assert_same(gpu, ref, f"frame {f}")
+    assert int(gpu.counters()["frame_cycles"][0]) > 100000
+
+
+def test_hotspot_mid_instruction_switch(fa, orc):
+    """Fetches through the F8 hotspots switch banks mid-instruction
+    (src/atari.rs:504-507): the phantom read of NOP at $FFF7 selects bank 0,
+    the JMP opcode comes from bank 0 at $FFF8 and its operand fetch at $FFF9
+    switches back to bank 1, so the target mixes both banks ($FF80, not
+    $FF11)."""
+    rom = bytearray([0xEA] * 8192)
+    for b in range(2):
+        o = b * 4096
+        rom[o: o + 6] = bytes([0xA2, 0xFF, 0x9A, 0x4C, 0xF0, 0xFF])      # LDX #$FF; TXS; JMP $FFF0
+        rom[o + 0xFF8: o + 0xFFB] = bytes([0x4C, 0x11, 0xFF] if b == 0 else [0xEA, 0x80, 0xFF])
+        rom[o + 0xFFC: o + 0x1000] = bytes([0x00, 0xF0, 0x00, 0xF0])
+        rom[o + 0xF80: o + 0xF83] = bytes([0x4C, 0x00, 0xF1])             # JMP $F100
+        rom[o + 0xF11: o + 0xF16] = bytes([0xE6, 0x90, 0x4C, 0x00, 0xF1])  # INC $90; JMP $F100
+        rom[o + 0x100: o + 0x10D] = bytes([0xA9, 0x02, 0x85, 0x00, 0xE6, 0x80 + b, 0xA9, 0x00, 0x85, 0x00,
+                                           0x4C, 0x00, 0xF0])
+    gpu, ref = _pair(fa, orc, bytes(rom), 64)
+    for f in range(3):
+        a = np.full(64, 1, np.uint8)
+        gpu.step(a)
+        ref.run(a[None, :])
+        assert_same(gpu, ref, f"frame {f}")
+    assert ref.states()[0, 8 + 0x10] == 0          # $90 untouched: JMP target came from bank 1
+
+
+def test_bad_lengths_raise(fa, W):
+    gpu = fa.BatchAtariGpu(W.rom_for(1), 8)
+    gpu.reset()
+    with pytest.raises(ValueError):
+        gpu.step(np.zeros(7, np.uint8))
+    with pytest.raises(ValueError):
+        gpu.step_multi(np.zeros(8 * 3, np.uint8), 2)
+    with pytest.raises(ValueError):
+        fa.BatchAtariGpu(bytes(5000), 8)
+    st = gpu.download_states_bytes()
+    st[0, 152] = 1                                                   # clock not a multiple of 3
+    with pytest.raises(fa.FastAtariError):
+        gpu.upload_states(st)
+    assert gpu.num_envs() == 8
