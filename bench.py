@@ -34,6 +34,25 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+WORKLOADS = {"fixed4k": 1, "f8": 2, "branchy16k": 3}
+WORKLOAD_NAMES = {1: "Fixed 4KB WSYNC ROM, per-frame step",
+                  2: "F8 8KB WSYNC ROM (bank switch every 16 lines), per-frame step",
+                  3: "F6 16KB no-WSYNC branchy ROM (all 151 opcodes), per-frame step"}
+
+
+def calibrate_copy(mib):
+    """Copy kernel of known size: FETCH_SIZE / WRITE_SIZE of this dispatch
+    scale the counters read for the frame kernel."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.arange(n, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4, "dispatches": 3}), flush=True)
+
+
 def cpu_baseline(rom, n_sample, frames, seed, threads):
     from oracle import oracle
     from fastatari import workloads as W
@@ -54,7 +73,14 @@ def main():
     ap.add_argument("--sorted", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="f8",
+                    help="f8: the headline 8 KB F8 WSYNC ROM; fixed4k / branchy16k: the other ROM classes")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy kernel of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(envs are independent; sharding is a per-device split of the env range).", file=sys.stderr)
@@ -63,13 +89,14 @@ def main():
     import fastatari
     from fastatari import workloads as W
 
-    seed = W.config_seed(2)
-    rom = W.rom_for(2)
+    cfg = WORKLOADS[args.workload]
+    seed = W.config_seed(cfg)
+    rom = W.rom_for(cfg)
     n = args.envs
     steps, warm, k = args.steps, args.warmup, args.k
     steps = max(k, (steps // k) * k)
 
-    log(f"bench: F8 WSYNC ROM, {n} envs, warmup {warm}, timed {steps} frames (plain) + {steps} frames (K={k})")
+    log(f"bench: {WORKLOAD_NAMES[cfg]}, {n} envs, warmup {warm}, timed {steps} frames (plain) + {steps} frames (K={k})")
     acts = W.ActionStream(seed, n).frames(warm + 2 * steps)            # [frames, n]
     d_act = fastatari.DeviceBuffer(acts.nbytes)
     d_act.upload(acts)
@@ -131,8 +158,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u8",
-        "data": "synthetic (seeded 8 KB F8 ROM, SplitMix64 per-env actions)",
-        "config": {"workload": "F8 8KB WSYNC ROM (bank switch every 16 lines), per-frame step",
+        "data": f"synthetic (seeded {len(rom) // 1024} KB ROM, SplitMix64 per-env actions)",
+        "config": {"workload": WORKLOAD_NAMES[cfg],
                    "envs": n, "frames": steps, "sorted": bool(args.sorted), "block": gpu.block_size()},
         "batched": {"k": k, "fps": round(multi_fps, 1), "ms_per_launch": round(multi_wall / (steps // k) * 1e3, 4),
                     "kernel_ms_per_launch": round(multi_ms_total / max(multi_launches, 1), 4)},
