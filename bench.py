@@ -160,7 +160,8 @@ def main():
         "dtype": "u8",
         "data": f"synthetic (seeded {len(rom) // 1024} KB ROM, SplitMix64 per-env actions)",
         "config": {"workload": WORKLOAD_NAMES[cfg],
-                   "envs": n, "frames": steps, "sorted": bool(args.sorted), "block": gpu.block_size()},
+                   "envs": n, "frames": steps, "sorted": bool(args.sorted), "block": gpu.block_size(),
+                   "max_lds_per_block": gpu.max_lds()},
         "batched": {"k": k, "fps": round(multi_fps, 1), "ms_per_launch": round(multi_wall / (steps // k) * 1e3, 4),
                     "kernel_ms_per_launch": round(multi_ms_total / max(multi_launches, 1), 4)},
         "api_level_fps": round(api_fps, 1),

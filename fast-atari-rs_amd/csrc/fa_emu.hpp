@@ -9,11 +9,19 @@
 //   bus            src/atari.rs:459-553 (HeadlessBus read/write decode)
 //   frame loop     src/atari.rs:584-614
 //
+// Instructions run through one shared pipeline (fetch, descriptor decode,
+// effective address, data read, compute, write-back) steered by the per-opcode
+// descriptors of fa_decode.hpp with predicated selects, so lanes executing
+// different opcodes stay on the same code path; only TIA / RIOT register
+// accesses, decimal-mode arithmetic and stack / jump instructions branch.
+//
 // LDS layout of one workgroup (byte offsets into fa_lds):
-//   [0, rom_bytes)                ROM image: the 4 KB window of a Fixed ROM,
+//   [0, 2048)                     opcode descriptors (256 x 8 bytes)
+//   [2048, 2048 + rom_bytes)      ROM image: the 4 KB window of a Fixed ROM,
 //                                 or every 4 KB bank of an F8/F6/F4 ROM
-//   [rom_bytes, + bs * 222)       the workgroup's AtariStateGpu images (AoS,
+//   [.., + bs * 222)              the workgroup's AtariStateGpu images (AoS,
 //                                 222-byte stride), staged with 16-byte loads
+//   [.., + bs)                    one discard byte per lane (predicated stores)
 // Hot fields (registers, TIA timing, RIOT timer, bank) live in VGPRs for the
 // whole launch; RAM and the rarely touched TIA/RIOT registers stay in the
 // lane's LDS image.
@@ -22,7 +30,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "fa_decode.hpp"
+
 namespace fa {
+
+constexpr uint32_t DESC0 = 0;        // LDS offset of the descriptor table
+constexpr uint32_t ROM0 = 2048;      // LDS offset of the ROM image
 
 // Byte offsets inside the 222-byte AtariStateGpu (src/cuda_env.rs:12-38).
 enum : uint32_t {
@@ -54,6 +67,9 @@ extern __shared__ __attribute__((aligned(16))) uint8_t fa_lds[];
 
 __device__ __forceinline__ uint32_t ld8(uint32_t off) { return fa_lds[off]; }
 __device__ __forceinline__ void st8(uint32_t off, uint32_t v) { fa_lds[off] = (uint8_t)v; }
+__device__ __forceinline__ uint32_t ld32(uint32_t off) {
+    return *reinterpret_cast<const uint32_t *>(fa_lds + off);
+}
 __device__ __forceinline__ uint32_t ld16(uint32_t off) {
     return (uint32_t)fa_lds[off] | ((uint32_t)fa_lds[off + 1] << 8);
 }
@@ -71,13 +87,14 @@ struct Emu {
     uint32_t tval, tps, tsel, tsh, tuf;
     // cartridge
     uint32_t bank;
-    // addressing: LDS offset of this lane's image; hotspot window
-    uint32_t img;
+    // addressing: LDS offsets of this lane's image and discard byte; hotspot window
+    uint32_t img, junk;
     uint32_t hot_lo, hot_n;
 
     // ------------------------------------------------------------ state I/O
-    __device__ __forceinline__ void load(uint32_t img_off, uint32_t hot_lo_, uint32_t hot_n_) {
+    __device__ __forceinline__ void load(uint32_t img_off, uint32_t junk_off, uint32_t hot_lo_, uint32_t hot_n_) {
         img = img_off;
+        junk = junk_off;
         hot_lo = hot_lo_;
         hot_n = hot_n_;
         a = ld8(img + S_A); x = ld8(img + S_X); y = ld8(img + S_Y); sp = ld8(img + S_SP);
@@ -115,6 +132,11 @@ struct Emu {
     // Pia::tick_n (src/pia.rs:107-166) with the prescaler division done by
     // shifts (select is 1, 8, 64 or 1024).
     __device__ __forceinline__ void pia_tick(uint32_t n) {
+        const bool post = tval == 0 && tuf;
+        if (!post && tps > n) { tps -= n; return; }       // common case: inside a prescaler period
+        pia_tick_slow(n);
+    }
+    __device__ __forceinline__ void pia_tick_slow(uint32_t n) {
         if (n == 0) return;
         if (tval == 0 && tuf) {
             tval = (0x100u - n) & 0xFFu;
@@ -267,416 +289,263 @@ struct Emu {
     }
 
     // ------------------------------------------------------------ bus
-    __device__ __forceinline__ void hotspot(uint32_t lo12) {
-        uint32_t d = lo12 - hot_lo;
-        if (d < hot_n) bank = d;
-    }
     __device__ __forceinline__ uint32_t ram_off(uint32_t a) const { return img + S_RAM + (a & 0x7Fu); }
 
-    // HeadlessBus::read (src/atari.rs:497-520)
+    // A ROM access at a hotspot switches the bank (HeadlessBus::check_bankswitch,
+    // src/atari.rs:459-492); hot_n = 0 for Fixed ROMs.
+    __device__ __forceinline__ void hotspot_if(bool pred, uint32_t a) {
+        const uint32_t hd = (a & 0xFFFu) - hot_lo;
+        bank = (pred && (a & 0x1000u) && hd < hot_n) ? hd : bank;
+    }
+    // TIA / RIOT I/O and the flat-array padding (src/atari.rs:509-519)
+    __device__ __forceinline__ uint32_t io_read(uint32_t a) {
+        if ((a & 0x280u) == 0x280u) return pia_read(a);
+        if (!(a & 0x80u)) return tia_read(a);
+        return 0u;
+    }
+    __device__ __forceinline__ void io_write(uint32_t a, uint32_t v) {
+        if (a & 0x200u) pia_write(a, v);
+        else tia_write(a, v);
+    }
+    // HeadlessBus::read (src/atari.rs:497-520): ROM window and the two RAM
+    // mirrors come from LDS, everything else from the TIA / RIOT.
     __device__ __forceinline__ uint32_t bus_read(uint32_t addr) {
-        uint32_t a = addr & 0x1FFFu;
-        if (a & 0x1000u) {
-            hotspot(a & 0xFFFu);
-            return ld8((bank << 12) + (a & 0xFFFu));
-        }
-        if (a & 0x80u) {
-            if (!(a & 0x200u)) return a < 0x200u ? ld8(ram_off(a)) : 0u;
-            return pia_read(a);
-        }
-        return tia_read(a);
+        const uint32_t a = addr & 0x1FFFu;
+        const bool rom = (a & 0x1000u) != 0u;
+        hotspot_if(true, a);
+        const bool ram = (a & 0x1E80u) == 0x0080u;
+        uint32_t v = ld8(rom ? ROM0 + (bank << 12) + (a & 0xFFFu) : ram_off(a));
+        if (!rom && !ram) v = io_read(a);
+        return v;
     }
     // Side effects of a read whose value is discarded: a hotspot switches the
-    // bank, INTIM clears the underflow flag.  RAM, TIA and the other RIOT
-    // registers have none.
-    __device__ __forceinline__ void touch(uint32_t addr) {
-        uint32_t a = addr & 0x1FFFu;
-        if (a & 0x1000u) hotspot(a & 0xFFFu);
-        else if ((a & 0x287u) == 0x284u) tuf = 0;
+    // bank, INTIM clears the underflow flag; nothing else has any.
+    __device__ __forceinline__ void touch_if(bool pred, uint32_t addr) {
+        const uint32_t a = addr & 0x1FFFu;
+        hotspot_if(pred, a);
+        tuf = (pred && (a & 0x1287u) == 0x284u) ? 0u : tuf;
     }
     // HeadlessBus::write (src/atari.rs:523-552)
     __device__ __forceinline__ void bus_write(uint32_t addr, uint32_t v) {
-        uint32_t a = addr & 0x1FFFu;
-        if (a & 0x1000u) { hotspot(a & 0xFFFu); return; }
-        if ((a & 0x280u) == 0x80u) { st8(ram_off(a), v); return; }
-        if (a & 0x200u) { pia_write(a, v); return; }
-        tia_write(a, v);
+        const uint32_t a = addr & 0x1FFFu;
+        const bool rom = (a & 0x1000u) != 0u;
+        hotspot_if(true, a);
+        const bool ram = (a & 0x1280u) == 0x0080u;
+        st8(ram ? ram_off(a) : junk, v);
+        if (!rom && !ram) io_write(a, v);
     }
-    // Zero page ($00-$FF): bit 7 selects RAM, otherwise the TIA.
+    // Zero page and stack page: bit 7 selects RAM, otherwise the TIA.
     __device__ __forceinline__ uint32_t zp_read(uint32_t a8) {
-        return (a8 & 0x80u) ? ld8(ram_off(a8)) : tia_read(a8);
+        uint32_t v = ld8(ram_off(a8));
+        if (!(a8 & 0x80u)) v = tia_read(a8);
+        return v;
     }
     __device__ __forceinline__ void zp_write(uint32_t a8, uint32_t v) {
-        if (a8 & 0x80u) st8(ram_off(a8), v);
-        else tia_write(a8, v);
+        st8((a8 & 0x80u) ? ram_off(a8) : junk, v);
+        if (!(a8 & 0x80u)) tia_write(a8, v);
     }
-    // Stack page ($100 | sp) decodes like the zero page.
     __device__ __forceinline__ void push(uint32_t v) { zp_write(sp, v); sp = (sp - 1u) & 0xFFu; }
     __device__ __forceinline__ uint32_t pull() { sp = (sp + 1u) & 0xFFu; return zp_read(sp); }
 
-    // ------------------------------------------------------------ ALU
+    // ------------------------------------------------------------ fetch
+    // Opcode and operand bytes plus the descriptor.  Inside the ROM window
+    // and below the hotspot / vector area the fetches have no side effects
+    // and come straight from LDS; elsewhere they go through the bus in the
+    // reference's order (opcode, then operands or the phantom read of PC).
+    __device__ __forceinline__ void fetch(uint32_t &b1, uint32_t &b2, uint32_t &d0, uint32_t &d1, uint32_t &op) {
+        if ((pc & 0x1000u) && (pc & 0xFFFu) < 0xFF0u) {
+            const uint32_t o = ROM0 + (bank << 12) + (pc & 0xFFFu);
+            op = ld8(o);
+            b1 = ld8(o + 1u);
+            b2 = ld8(o + 2u);
+            d0 = ld32(DESC0 + op * 8u);
+            d1 = ld32(DESC0 + op * 8u + 4u);
+        } else {
+            op = bus_read(pc);
+            d0 = ld32(DESC0 + op * 8u);
+            d1 = ld32(DESC0 + op * 8u + 4u);
+            const uint32_t len = (d0 >> D_LEN_SHIFT) & 3u;
+            const bool illegal = (d0 & 3u) == CLS_ILLEGAL;
+            b1 = len >= 2u ? bus_read((pc + 1u) & 0xFFFFu) : 0u;
+            touch_if(len == 1u && !illegal, (pc + 1u) & 0xFFFFu);
+            b2 = len == 3u ? bus_read((pc + 2u) & 0xFFFFu) : 0u;
+        }
+    }
+
+    // ------------------------------------------------------------ execute
     __device__ __forceinline__ void set_zn(uint32_t v) {
         p = (p & ~(FZ | FN)) | (v & 0x80u) | (v == 0u ? FZ : 0u);
     }
-    __device__ __forceinline__ void setf(uint32_t f, bool on) { p = on ? (p | f) : (p & ~f); }
+    __device__ __forceinline__ uint32_t reg(uint32_t sel) const {
+        return sel == R_A ? a : sel == R_X ? x : sel == R_Y ? y : sp;
+    }
+    __device__ __forceinline__ void set_reg(uint32_t sel, uint32_t v) {
+        a = sel == R_A ? v : a;
+        x = sel == R_X ? v : x;
+        y = sel == R_Y ? v : y;
+        sp = sel == R_SP ? v : sp;
+    }
 
-    // ADC, src/cpu.rs:152-175
-    __device__ __forceinline__ void adc(uint32_t v) {
-        uint32_t c = p & FC;
-        if (p & FD) {
-            uint32_t lo = (a & 0xFu) + (v & 0xFu) + c;
+    // Decimal-mode ADC / SBC accumulator and flags (src/cpu.rs:155-166, 180-191),
+    // given the binary-mode operand `v` and the pre-instruction A and P.
+    __device__ __forceinline__ void decimal(bool sub, uint32_t a0, uint32_t v, uint32_t p0) {
+        const uint32_t c = p0 & FC;
+        if (!sub) {
+            uint32_t lo = (a0 & 0xFu) + (v & 0xFu) + c;
             if (lo > 9u) lo += 6u;
-            uint32_t hi = (a >> 4) + (v >> 4) + (lo > 0xFu ? 1u : 0u);
-            uint32_t bin = (a + v + c) & 0xFFu;
-            uint32_t inter = ((hi << 4) | (lo & 0xFu)) & 0xFFu;
-            uint32_t np = p & ~(FZ | FN | FV | FC);
-            np |= bin == 0u ? FZ : 0u;
+            uint32_t hi = (a0 >> 4) + (v >> 4) + (lo > 0xFu ? 1u : 0u);
+            const uint32_t inter = ((hi << 4) | (lo & 0xFu)) & 0xFFu;
+            uint32_t np = p & ~(FN | FV | FC);           // Z stays the binary one
             np |= (hi & 8u) ? FN : 0u;
-            np |= (~(a ^ v) & (a ^ inter) & 0x80u) ? FV : 0u;
+            np |= (~(a0 ^ v) & (a0 ^ inter) & 0x80u) ? FV : 0u;
             if (hi > 9u) hi += 6u;
             np |= hi > 0xFu ? FC : 0u;
             p = np;
             a = ((hi & 0xFu) << 4) | (lo & 0xFu);
         } else {
-            uint32_t s = a + v + c;
-            uint32_t r = s & 0xFFu;
-            uint32_t np = p & ~(FZ | FN | FV | FC);
-            np |= s > 0xFFu ? FC : 0u;
-            np |= (~(a ^ v) & (a ^ r) & 0x80u) ? FV : 0u;
-            a = r;
-            p = np;
-            set_zn(a);
-        }
-    }
-    // SBC, src/cpu.rs:177-200
-    __device__ __forceinline__ void sbc(uint32_t v) {
-        uint32_t borrow = (p & FC) ? 0u : 1u;
-        int32_t d = (int32_t)a - (int32_t)v - (int32_t)borrow;
-        uint32_t r = (uint32_t)d & 0xFFu;
-        uint32_t np = p & ~(FV | FC);
-        np |= d >= 0 ? FC : 0u;
-        np |= ((a ^ v) & (a ^ r) & 0x80u) ? FV : 0u;
-        if (p & FD) {
-            int32_t lo = (int32_t)(a & 0xFu) - (int32_t)(v & 0xFu) - (int32_t)borrow;
-            uint32_t lo_b = 0;
-            if (lo < 0) { lo -= 6; lo_b = 1; }
-            int32_t hi = (int32_t)(a >> 4) - (int32_t)(v >> 4) - (int32_t)lo_b;
-            if (hi < 0) hi -= 6;
-            p = np;
-            set_zn(r);
+            // all flags are the binary ones; only the accumulator differs
+            const uint32_t borrow = c ^ 1u;
+            int32_t lo = (int32_t)(a0 & 0xFu) - (int32_t)(v & 0xFu) - (int32_t)borrow;
+            const uint32_t lb = lo < 0 ? 1u : 0u;
+            lo -= lb ? 6 : 0;
+            int32_t hi = (int32_t)(a0 >> 4) - (int32_t)(v >> 4) - (int32_t)lb;
+            hi -= hi < 0 ? 6 : 0;
             a = (((uint32_t)hi & 0xFu) << 4) | ((uint32_t)lo & 0xFu);
-        } else {
+        }
+    }
+
+    // Executes one decoded instruction (pc already points past it).  Returns
+    // the cycle count, or 0 for an illegal opcode (src/cpu.rs:1139-1141).
+    __device__ __forceinline__ uint32_t execute(uint32_t b1, uint32_t b2, uint32_t d0, uint32_t d1) {
+        uint32_t cyc = (d0 >> D_CYC_SHIFT) & 7u;
+        const uint32_t cls = d0 & 3u;
+        if (cls == CLS_COMPUTE) {
+            const uint32_t kind = (d0 >> D_KIND_SHIFT) & 7u;
+            const uint32_t s = reg((d1 >> D1_SRC_SHIFT) & 7u);
+            uint32_t v = (d0 & D_IMM) ? b1 : s;
+            uint32_t ea = 0;
+            if (d0 & D_MEM) {
+                const uint32_t ind = (d0 >> D_IND_SHIFT) & 3u;
+                uint32_t ixsel = (d0 >> D_IDX_SHIFT) & 3u;
+                uint32_t base = (d0 & D_ZP) ? b1 : (b1 | (b2 << 8));
+                if (ind) {
+                    // pointer fetch from the zero page (no side effects)
+                    const uint32_t zp = ind == 1u ? ((b1 + x) & 0xFFu) : b1;
+                    base = zp_read(zp) | (zp_read((zp + 1u) & 0xFFu) << 8);
+                    ixsel = ind == 2u ? 2u : 0u;
+                }
+                const uint32_t idx = ixsel == 1u ? x : (ixsel == 2u ? y : 0u);
+                ea = (base + idx) & ((d0 & D_ZP) ? 0xFFu : 0xFFFFu);
+                const bool cross = ((base ^ ea) & 0xFF00u) != 0u;
+                const bool fix = (d0 & D_DUMMY_ALWAYS) || ((d0 & D_DUMMY_CROSS) && cross);
+                touch_if(fix, (base & 0xFF00u) | (ea & 0xFFu));
+                cyc += ((d0 & D_DUMMY_CROSS) && cross) ? 1u : 0u;
+                if (d0 & D_RD) v = bus_read(ea);
+            }
+            const uint32_t p0 = p, a0 = a;
+            // candidates
+            const uint32_t mand = (d0 & D_M_AND) ? 0xFFu : 0u, mxor = (d0 & D_M_XOR) ? 0xFFu : 0u,
+                           mld = (d0 & D_M_LD) ? 0xFFu : 0u;
+            const uint32_t lg = ((a & v) & mand) | ((a ^ v) & mxor) | (v & mld);
+            const uint32_t vv = (d0 & D_INV) ? (v ^ 0xFFu) : v;
+            const uint32_t sum = s + vv + ((d0 & D_CIN1) ? 1u : (p & FC));
+            const uint32_t ar = sum & 0xFFu;
+            const uint32_t rop = (d1 >> D1_RMW_SHIFT) & 7u;
+            const uint32_t cin = p & FC;
+            const uint32_t shl = ((v << 1) | (rop == 1u ? cin : 0u)) & 0xFFu;
+            const uint32_t shr = (v >> 1) | (rop == 3u ? (cin << 7) : 0u);
+            const uint32_t idc = (v + (rop == 5u ? 1u : 0xFFu)) & 0xFFu;
+            const uint32_t rr = rop < 2u ? shl : (rop < 4u ? shr : idc);
+            const uint32_t dl = (d1 >> D1_DELTA_SHIFT) & 3u;
+            const uint32_t mv = (s + (dl == 1u ? 1u : (dl == 2u ? 0xFFu : 0u))) & 0xFFu;
+            uint32_t r = kind == K_LOGIC ? lg : kind == K_ARITH ? ar : kind == K_RMW ? rr : kind == K_MOVE ? mv : s;
+            // flags
+            const uint32_t c_new = kind == K_ARITH ? (sum >> 8) & 1u : (rop < 2u ? (v >> 7) : (v & 1u));
+            const uint32_t v_new = (~(s ^ vv) & (s ^ ar) & 0x80u) ? FV : 0u;
+            uint32_t np = p;
+            np = (d0 & D_SETNZ) ? ((np & ~(FN | FZ)) | (r & 0x80u) | (r == 0u ? FZ : 0u)) : np;
+            np = (d0 & D_SETC) ? ((np & ~FC) | c_new) : np;
+            np = (d0 & D_SETV) ? ((np & ~FV) | v_new) : np;
+            np = kind == K_BIT ? ((np & ~(FN | FV | FZ)) | (v & 0xC0u) | ((a & v) == 0u ? FZ : 0u)) : np;
+            const uint32_t fm = (d1 >> D1_FMASK_SHIFT) & 0xFFu, fv = d1 >> D1_FVAL_SHIFT;
+            np = kind == K_FLAGS ? ((np & ~fm) | fv) : np;
             p = np;
-            a = r;
-            set_zn(a);
+            set_reg((d1 >> D1_DST_SHIFT) & 7u, r);
+            if ((d0 & D_DEC) && (p0 & FD)) decimal((d0 & D_INV) != 0u, a0, v, p0);
+            if (d0 & D_RMWMEM) bus_write(ea, v);
+            if (d0 & D_WR) bus_write(ea, r);
+            return cyc;
         }
-    }
-    __device__ __forceinline__ void cmp(uint32_t reg, uint32_t v) {
-        setf(FC, reg >= v);
-        set_zn((reg - v) & 0xFFu);
-    }
-
-    // Read operations of every read mode (aaa / cc opcode fields).
-    __device__ __forceinline__ void read_op(uint32_t op, uint32_t v) {
-        uint32_t cc = op & 3u, aaa = op >> 5;
-        if (cc == 1u) {
-            switch (aaa) {
-            case 0: a |= v; set_zn(a); break;
-            case 1: a &= v; set_zn(a); break;
-            case 2: a ^= v; set_zn(a); break;
-            case 3: adc(v); break;
-            case 5: a = v; set_zn(a); break;
-            case 6: cmp(a, v); break;
-            case 7: sbc(v); break;
-            default: break;
-            }
-        } else if (cc == 2u) {
-            x = v; set_zn(x);
-        } else {
-            switch (aaa) {
-            case 1: p = (p & ~(FZ | FV | FN)) | ((a & v) == 0u ? FZ : 0u) | (v & 0xC0u); break;   // BIT
-            case 5: y = v; set_zn(y); break;
-            case 6: cmp(y, v); break;
-            default: cmp(x, v); break;                                                            // CPX
-            }
+        if (cls == CLS_BRANCH) {
+            const uint32_t fm = (d1 >> D1_FMASK_SHIFT) & 0xFFu;
+            const bool taken = ((p & fm) != 0u) == ((d1 >> D1_FVAL_SHIFT) != 0u);
+            const uint32_t np = (pc + b1 - ((b1 & 0x80u) << 1)) & 0xFFFFu;
+            const bool cross = ((np ^ pc) & 0xFF00u) != 0u;
+            touch_if(taken, pc);                                    // phantom read of PC
+            touch_if(taken && cross, (pc & 0xFF00u) | (np & 0xFFu)); // before the high-byte fix
+            cyc += taken ? (cross ? 2u : 1u) : 0u;
+            pc = taken ? np : pc;
+            return cyc;
         }
-    }
-    __device__ __forceinline__ uint32_t store_reg(uint32_t op) const {
-        uint32_t cc = op & 3u;
-        return cc == 1u ? a : (cc == 2u ? x : y);
-    }
-    __device__ __forceinline__ uint32_t rmw_op(uint32_t op, uint32_t v) {
-        uint32_t r;
-        switch (op >> 5) {
-        case 0: setf(FC, v & 0x80u); r = (v << 1) & 0xFFu; break;                 // ASL
-        case 1: r = ((v << 1) | (p & FC)) & 0xFFu; setf(FC, v & 0x80u); break;    // ROL
-        case 2: setf(FC, v & 1u); r = v >> 1; break;                               // LSR
-        case 3: r = (v >> 1) | ((p & FC) << 7); setf(FC, v & 1u); break;         // ROR
-        case 6: r = (v - 1u) & 0xFFu; break;                                       // DEC
-        default: r = (v + 1u) & 0xFFu; break;                                      // INC
-        }
-        set_zn(r);
-        return r;
-    }
-    __device__ __forceinline__ bool branch_taken(uint32_t op) const {
-        // bits 7-6 select N, V, C, Z; bit 5 is the value tested for
-        constexpr uint32_t kFlag = (FN << 0) | (FV << 8) | (FC << 16) | (FZ << 24);
-        uint32_t f = (kFlag >> ((op >> 6) * 8u)) & 0xFFu;
-        return ((p & f) != 0u) == ((op & 0x20u) != 0u);
-    }
-
-    // ------------------------------------------------------------ CPU step
-    // Executes the instruction at pc.  With FAST, the opcode and both operand
-    // bytes were prefetched from the current bank (pc is inside the ROM
-    // window, away from the hotspots, so those reads and the phantom read of
-    // PC have no side effect).  Returns the cycle count, or 0 for an illegal
-    // opcode (PC already advanced past it, src/cpu.rs:1139-1141).
-    template <bool FAST>
-    __device__ __forceinline__ uint32_t opnd(uint32_t pre) {
-        uint32_t v = FAST ? pre : bus_read(pc);
-        pc = (pc + 1u) & 0xFFFFu;
-        return v;
-    }
-    template <bool FAST>
-    __device__ __forceinline__ void phantom_pc() {
-        if (!FAST) touch(pc);
-    }
-
-    template <bool FAST>
-    __device__ __forceinline__ uint32_t step(uint32_t op, uint32_t b1, uint32_t b2) {
-        pc = (pc + 1u) & 0xFFFFu;
-        uint32_t v, ad, eff;
-        switch (op) {
-        // implied / accumulator (src/cpu.rs:257-330)
-        case 0xEA: phantom_pc<FAST>(); return 2;
-        case 0x18: case 0x38: case 0x58: case 0x78: case 0xB8: case 0xD8: case 0xF8: {
-            phantom_pc<FAST>();
-            // bits 7-6 pick C, I, V, D; bit 5 the value (CLV has no set form)
-            constexpr uint32_t kF = FC | (FI << 8) | (FV << 16) | (FD << 24);
-            uint32_t f = (kF >> ((op >> 6) * 8u)) & 0xFFu;
-            setf(f, (op & 0x20u) && op != 0xB8u);
-            return 2;
-        }
-        case 0xAA: phantom_pc<FAST>(); x = a; set_zn(x); return 2;
-        case 0xA8: phantom_pc<FAST>(); y = a; set_zn(y); return 2;
-        case 0x8A: phantom_pc<FAST>(); a = x; set_zn(a); return 2;
-        case 0x98: phantom_pc<FAST>(); a = y; set_zn(a); return 2;
-        case 0xBA: phantom_pc<FAST>(); x = sp; set_zn(x); return 2;
-        case 0x9A: phantom_pc<FAST>(); sp = x; return 2;
-        case 0xE8: phantom_pc<FAST>(); x = (x + 1u) & 0xFFu; set_zn(x); return 2;
-        case 0xC8: phantom_pc<FAST>(); y = (y + 1u) & 0xFFu; set_zn(y); return 2;
-        case 0xCA: phantom_pc<FAST>(); x = (x - 1u) & 0xFFu; set_zn(x); return 2;
-        case 0x88: phantom_pc<FAST>(); y = (y - 1u) & 0xFFu; set_zn(y); return 2;
-        case 0x0A: case 0x2A: case 0x4A: case 0x6A: phantom_pc<FAST>(); a = rmw_op(op, a); return 2;
-
-        // immediate (src/cpu.rs:336-356)
-        case 0x69: case 0x29: case 0xC9: case 0xE0: case 0xC0: case 0x49:
-        case 0xA9: case 0xA2: case 0xA0: case 0x09: case 0xE9:
-            v = opnd<FAST>(b1);
-            read_op(op, v);
-            return 2;
-
-        // zero page read / write / RMW (src/cpu.rs:362-410, 498-521)
-        case 0xA5: case 0xA6: case 0xA4: case 0x65: case 0x25: case 0xC5:
-        case 0xE4: case 0xC4: case 0x45: case 0x05: case 0xE5: case 0x24:
-            ad = opnd<FAST>(b1);
-            read_op(op, zp_read(ad));
-            return 3;
-        case 0x85: case 0x86: case 0x84:
-            ad = opnd<FAST>(b1);
-            zp_write(ad, store_reg(op));
-            return 3;
-        case 0x06: case 0x46: case 0x26: case 0x66: case 0xE6: case 0xC6:
-            ad = opnd<FAST>(b1);
-            v = zp_read(ad);
-            if (!(ad & 0x80u)) tia_write(ad, v);                 // phantom write (RAM: same value)
-            zp_write(ad, rmw_op(op, v));
-            return 5;
-
-        // zero page indexed (src/cpu.rs:415-493, 526-553); the phantom read of
-        // the base address is RAM or TIA and has no side effect
-        case 0xB5: case 0xB4: case 0x75: case 0x35: case 0xD5: case 0x55:
-        case 0x15: case 0xF5: case 0xB6:
-            ad = (opnd<FAST>(b1) + (op == 0xB6u ? y : x)) & 0xFFu;
-            read_op(op, zp_read(ad));
-            return 4;
-        case 0x95: case 0x94: case 0x96:
-            ad = (opnd<FAST>(b1) + (op == 0x96u ? y : x)) & 0xFFu;
-            zp_write(ad, store_reg(op));
-            return 4;
-        case 0x16: case 0x56: case 0x36: case 0x76: case 0xF6: case 0xD6:
-            ad = (opnd<FAST>(b1) + x) & 0xFFu;
-            v = zp_read(ad);
-            if (!(ad & 0x80u)) tia_write(ad, v);
-            zp_write(ad, rmw_op(op, v));
-            return 6;
-
-        // absolute (src/cpu.rs:559-644)
-        case 0xAD: case 0xAE: case 0xAC: case 0x6D: case 0x2D: case 0xCD:
-        case 0xEC: case 0xCC: case 0x4D: case 0x0D: case 0xED: case 0x2C:
-            ad = opnd<FAST>(b1);
-            ad |= opnd<FAST>(b2) << 8;
-            read_op(op, bus_read(ad));
-            return 4;
-        case 0x8D: case 0x8E: case 0x8C:
-            ad = opnd<FAST>(b1);
-            ad |= opnd<FAST>(b2) << 8;
-            bus_write(ad, store_reg(op));
-            return 4;
-        case 0x0E: case 0x4E: case 0x2E: case 0x6E: case 0xEE: case 0xCE:
-            ad = opnd<FAST>(b1);
-            ad |= opnd<FAST>(b2) << 8;
-            v = bus_read(ad);
-            bus_write(ad, v);
-            bus_write(ad, rmw_op(op, v));
-            return 6;
-
-        // absolute indexed (src/cpu.rs:649-788)
-        case 0xBD: case 0xBC: case 0x7D: case 0x3D: case 0xDD: case 0x5D:
-        case 0x1D: case 0xFD:
-        case 0xB9: case 0xBE: case 0x79: case 0x39: case 0xD9: case 0x59:
-        case 0x19: case 0xF9: {
-            ad = opnd<FAST>(b1);
-            ad |= opnd<FAST>(b2) << 8;
-            uint32_t idx = ((op & 0x1Fu) == 0x19u || op == 0xBEu) ? y : x;
-            eff = (ad + idx) & 0xFFFFu;
-            uint32_t cross = (ad ^ eff) & 0xFF00u;
-            if (cross) touch((ad & 0xFF00u) | (eff & 0xFFu));
-            read_op(op, bus_read(eff));
-            return cross ? 5 : 4;
-        }
-        case 0x9D: case 0x99:
-            ad = opnd<FAST>(b1);
-            ad |= opnd<FAST>(b2) << 8;
-            eff = (ad + (op == 0x99u ? y : x)) & 0xFFFFu;
-            touch((ad & 0xFF00u) | (eff & 0xFFu));
-            bus_write(eff, a);
-            return 5;
-        case 0x1E: case 0x5E: case 0x3E: case 0x7E: case 0xFE: case 0xDE:
-            ad = opnd<FAST>(b1);
-            ad |= opnd<FAST>(b2) << 8;
-            eff = (ad + x) & 0xFFFFu;
-            touch((ad & 0xFF00u) | (eff & 0xFFu));
-            v = bus_read(eff);
-            bus_write(eff, v);
-            bus_write(eff, rmw_op(op, v));
-            return 7;
-
-        // (zp,X) (src/cpu.rs:793-841); base and pointer reads are zero page
-        case 0xA1: case 0x61: case 0x21: case 0xC1: case 0x41: case 0x01: case 0xE1: case 0x81: {
-            uint32_t zp = (opnd<FAST>(b1) + x) & 0xFFu;
-            ad = zp_read(zp) | (zp_read((zp + 1u) & 0xFFu) << 8);
-            if (op == 0x81u) bus_write(ad, a);
-            else read_op(op, bus_read(ad));
-            return 6;
-        }
-        // (zp),Y (src/cpu.rs:846-901)
-        case 0xB1: case 0x71: case 0x31: case 0xD1: case 0x51: case 0x11: case 0xF1: {
-            uint32_t zp = opnd<FAST>(b1);
-            ad = zp_read(zp) | (zp_read((zp + 1u) & 0xFFu) << 8);
-            eff = (ad + y) & 0xFFFFu;
-            uint32_t cross = (ad ^ eff) & 0xFF00u;
-            if (cross) touch((ad & 0xFF00u) | (eff & 0xFFu));
-            read_op(op, bus_read(eff));
-            return cross ? 6 : 5;
-        }
-        case 0x91: {
-            uint32_t zp = opnd<FAST>(b1);
-            ad = zp_read(zp) | (zp_read((zp + 1u) & 0xFFu) << 8);
-            eff = (ad + y) & 0xFFFFu;
-            touch((ad & 0xFF00u) | (eff & 0xFFu));
-            bus_write(eff, a);
-            return 6;
-        }
-
-        // branches (src/cpu.rs:906-944)
-        case 0x90: case 0xB0: case 0xF0: case 0x30: case 0xD0: case 0x10: case 0x50: case 0x70: {
-            uint32_t off = opnd<FAST>(b1);
-            if (!branch_taken(op)) return 2;
-            phantom_pc<FAST>();
-            uint32_t np = (pc + off - ((off & 0x80u) << 1)) & 0xFFFFu;
-            uint32_t c = 3;
-            if ((np ^ pc) & 0xFF00u) {
-                touch((pc & 0xFF00u) | (np & 0xFFu));
-                c = 4;
-            }
-            pc = np;
-            return c;
-        }
-
-        // JMP abs / JMP (ind) (src/cpu.rs:949-978)
-        case 0x4C: {
-            uint32_t lo = opnd<FAST>(b1);
-            uint32_t hi = FAST ? b2 : bus_read(pc);
+        if (cls == CLS_ILLEGAL) return 0u;
+        // stack / jumps (src/cpu.rs:949-1137); phantom stack reads have no side effect
+        switch ((d1 >> D1_SPEC_SHIFT) & 15u) {
+        case SP_JMP: pc = b1 | (b2 << 8); break;
+        case SP_JMPI: {
+            const uint32_t ptr = b1 | (b2 << 8);
+            const uint32_t lo = bus_read(ptr);
+            const uint32_t hi = bus_read((ptr & 0xFF00u) | ((ptr + 1u) & 0xFFu));
             pc = lo | (hi << 8);
-            return 3;
+            break;
         }
-        case 0x6C: {
-            ad = opnd<FAST>(b1);
-            ad |= opnd<FAST>(b2) << 8;
-            uint32_t lo = bus_read(ad);
-            uint32_t hi = bus_read((ad & 0xFF00u) | ((ad + 1u) & 0xFFu));
-            pc = lo | (hi << 8);
-            return 5;
+        case SP_JSR: {
+            const uint32_t ret = (pc - 1u) & 0xFFFFu;       // address of the high operand byte
+            push(ret >> 8);
+            push(ret & 0xFFu);
+            pc = b1 | (b2 << 8);
+            break;
         }
-        // JSR (src/cpu.rs:983-1005): the phantom stack read has no side effect
-        case 0x20: {
-            uint32_t lo = opnd<FAST>(b1);
-            push(pc >> 8);
-            push(pc & 0xFFu);
-            uint32_t hi = FAST ? b2 : bus_read(pc);
-            pc = lo | (hi << 8);
-            return 6;
-        }
-        // RTS (src/cpu.rs:1010-1028)
-        case 0x60: {
-            phantom_pc<FAST>();
-            uint32_t lo = pull();
-            uint32_t hi = pull();
-            ad = lo | (hi << 8);
-            touch(ad);
+        case SP_RTS: {
+            const uint32_t lo = pull();
+            const uint32_t hi = pull();
+            const uint32_t ad = lo | (hi << 8);
+            touch_if(true, ad);
             pc = (ad + 1u) & 0xFFFFu;
-            return 6;
+            break;
         }
-        // RTI (src/cpu.rs:1033-1051)
-        case 0x40: {
-            phantom_pc<FAST>();
+        case SP_RTI: {
             p = pull() & ~(FB | FU);
-            uint32_t lo = pull();
-            uint32_t hi = pull();
+            const uint32_t lo = pull();
+            const uint32_t hi = pull();
             pc = lo | (hi << 8);
-            return 6;
+            break;
         }
-        // BRK (src/cpu.rs:1056-1080)
-        case 0x00: {
-            phantom_pc<FAST>();
-            pc = (pc + 1u) & 0xFFFFu;
+        case SP_BRK: {
+            pc = (pc + 1u) & 0xFFFFu;                       // skip the padding byte
             push(pc >> 8);
             push(pc & 0xFFu);
             push(p | FB | FU);
             p |= FI;
-            uint32_t lo = bus_read(0xFFFEu);
-            uint32_t hi = bus_read(0xFFFFu);
+            const uint32_t lo = bus_read(0xFFFEu);
+            const uint32_t hi = bus_read(0xFFFFu);
             pc = lo | (hi << 8);
-            return 7;
+            break;
         }
-        // stack (src/cpu.rs:1085-1137)
-        case 0x48: phantom_pc<FAST>(); push(a); return 3;
-        case 0x08: phantom_pc<FAST>(); push(p | FB | FU); return 3;
-        case 0x68: phantom_pc<FAST>(); a = pull(); set_zn(a); return 4;
-        case 0x28: phantom_pc<FAST>(); p = pull() & ~(FB | FU); return 4;
-
-        default:
-            return 0;
+        case SP_PHA: push(a); break;
+        case SP_PHP: push(p | FB | FU); break;
+        case SP_PLA: a = pull(); set_zn(a); break;
+        default: p = pull() & ~(FB | FU); break;            // SP_PLP
         }
+        return cyc;
     }
 
-    // PC is inside the ROM window and pc+2 stays below the hotspot/vector
-    // area, so the opcode and operand fetches have no side effects.
-    __device__ __forceinline__ bool fast_fetch() const {
-        return (pc & 0x1000u) && ((pc & 0xFFFu) < 0xFF0u);
+    // Fetch, advance PC and execute one instruction.
+    __device__ __forceinline__ uint32_t exec() {
+        uint32_t b1, b2, d0, d1, op;
+        fetch(b1, b2, d0, d1, op);
+        pc = (pc + ((d0 >> D_LEN_SHIFT) & 3u)) & 0xFFFFu;
+        return execute(b1, b2, d0, d1);
     }
-    __device__ __forceinline__ uint32_t rom_off() const { return (bank << 12) + (pc & 0xFFFu); }
 };
 
 }  // namespace fa

@@ -41,6 +41,7 @@ struct StepArgs {
     const uint8_t *actions;     // [f * stride + i], or [i] with fs; NULL: no set_action
     const uint8_t *fs;          // per-env frame counts (frameskip), or NULL
     uint8_t *obs;               // n x 128, or NULL
+    const uint32_t *desc;       // 512 words: opcode descriptors (fa_decode.hpp)
     const uint8_t *rom;         // ROM image (rom_bytes, multiple of 4096)
     uint32_t rom_bytes;
     uint32_t hot_lo, hot_n;     // hotspot window (lo12 - hot_lo < hot_n)
@@ -50,32 +51,23 @@ struct StepArgs {
 // One CPU instruction for the active lanes.  Returns cycles (0 = illegal).
 template <bool SORTED>
 __device__ __forceinline__ uint32_t cpu_step(Emu &e) {
-    if (!SORTED) {
-        if (e.fast_fetch()) {
-            uint32_t o = e.rom_off();
-            return e.step<true>(ld8(o), ld8(o + 1u), ld8(o + 2u));
-        }
-        uint32_t op = e.bus_read(e.pc);
-        return e.step<false>(op, 0u, 0u);
-    }
-    // Opcode grouping.  The key is the opcode when the fetch has no side
-    // effects, else 0x100 (those lanes take the generic bus path together).
-    const bool fast = e.fast_fetch();
-    const uint32_t o = e.rom_off();
-    const uint32_t key = fast ? ld8(o) : 0x100u;
+    if (!SORTED) return e.exec();
+    // Opcode grouping: lanes are served one distinct opcode at a time; the
+    // leader's descriptor is read into SGPRs, so every select of the
+    // pipeline resolves as scalar control flow for the whole group.
+    uint32_t b1, b2, d0, d1, op;
+    e.fetch(b1, b2, d0, d1, op);
+    e.pc = (e.pc + ((d0 >> D_LEN_SHIFT) & 3u)) & 0xFFFFu;
     uint64_t rem = __ballot(1);
     uint32_t c = 0;
     while (rem) {
-        const uint32_t lead = (uint32_t)__builtin_ctzll(rem);
-        const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)lead);
-        const bool mine = key == lk;
+        const int lead = (int)__builtin_ctzll(rem);
+        const uint32_t lop = (uint32_t)__builtin_amdgcn_readlane((int)op, lead);
+        const bool mine = op == lop;
         if (mine) {
-            if (lk == 0x100u) {
-                uint32_t op = e.bus_read(e.pc);
-                c = e.step<false>(op, 0u, 0u);
-            } else {
-                c = e.step<true>(lk, ld8(o + 1u), ld8(o + 2u));
-            }
+            const uint32_t u0 = (uint32_t)__builtin_amdgcn_readlane((int)d0, lead);
+            const uint32_t u1 = (uint32_t)__builtin_amdgcn_readlane((int)d1, lead);
+            c = e.execute(b1, b2, u0, u1);
         }
         rem &= ~__ballot(mine);
     }
@@ -120,13 +112,17 @@ __global__ void __launch_bounds__(256) fa_frame_kernel(StepArgs g) {
     const uint32_t bs = blockDim.x;
     const uint32_t tid = threadIdx.x;
     const uint32_t first = blockIdx.x * bs;
-    const uint32_t img0 = g.rom_bytes;
+    const uint32_t img0 = ROM0 + g.rom_bytes;
     const uint32_t img_bytes = bs * STATE_BYTES;
 
-    // Stage the ROM and this workgroup's state block into LDS (16-byte loads).
+    // Stage the descriptors, the ROM and this workgroup's state block into
+    // LDS (16-byte loads).
     {
+        const uint4 *dsrc = reinterpret_cast<const uint4 *>(g.desc);
+        uint4 *ddst = reinterpret_cast<uint4 *>(fa_lds + DESC0);
+        for (uint32_t i = tid; i < 2048u / 16u; i += bs) ddst[i] = dsrc[i];
         const uint4 *src = reinterpret_cast<const uint4 *>(g.rom);
-        uint4 *dst = reinterpret_cast<uint4 *>(fa_lds);
+        uint4 *dst = reinterpret_cast<uint4 *>(fa_lds + ROM0);
         for (uint32_t i = tid; i < g.rom_bytes / 16u; i += bs) dst[i] = src[i];
         const uint4 *ssrc = reinterpret_cast<const uint4 *>(g.states + (size_t)first * STATE_BYTES);
         uint4 *sdst = reinterpret_cast<uint4 *>(fa_lds + img0);
@@ -137,7 +133,7 @@ __global__ void __launch_bounds__(256) fa_frame_kernel(StepArgs g) {
     const uint32_t idx = first + tid;
     if (idx < g.n) {
         Emu e;
-        e.load(img0 + tid * STATE_BYTES, g.hot_lo, g.hot_n);
+        e.load(img0 + tid * STATE_BYTES, img0 + img_bytes + tid, g.hot_lo, g.hot_n);
         uint32_t st = g.status[idx];
         uint32_t nfr = g.fs ? (uint32_t)g.fs[idx] : g.k;
         uint32_t cpu_cyc = 0, instr = 0, fcyc = 0;
@@ -231,6 +227,7 @@ struct fa_env {
     uint32_t block = 0;
     size_t max_lds = 65536;
 
+    uint32_t *d_desc = nullptr;
     uint8_t *d_rom = nullptr, *d_states = nullptr, *d_actions = nullptr, *d_fs = nullptr, *d_obs = nullptr;
     uint64_t *d_cpu = nullptr, *d_fcyc = nullptr, *d_instr = nullptr;
     uint32_t *d_status = nullptr;
@@ -245,10 +242,14 @@ struct fa_env {
 
 namespace {
 
+size_t lds_bytes(const fa_env *e, uint32_t bs) {
+    return fa::ROM0 + e->rom_bytes + (size_t)bs * (fa::STATE_BYTES + 1u);
+}
+
 uint32_t pick_block(const fa_env *e) {
     if (e->block) return e->block;
     for (uint32_t b : {256u, 128u, 64u})
-        if (e->rom_bytes + b * fa::STATE_BYTES <= e->max_lds) return b;
+        if (lds_bytes(e, b) <= e->max_lds) return b;
     return 64u;
 }
 
@@ -272,6 +273,7 @@ int launch_frames(fa_env *e, const uint8_t *d_actions, const uint8_t *d_fs, uint
     g.actions = d_actions;
     g.fs = d_fs;
     g.obs = d_obs;
+    g.desc = e->d_desc;
     g.rom = e->d_rom;
     g.rom_bytes = e->rom_bytes;
     g.hot_lo = e->hot_lo;
@@ -280,7 +282,7 @@ int launch_frames(fa_env *e, const uint8_t *d_actions, const uint8_t *d_fs, uint
     g.k = k;
     g.stride = (uint32_t)e->n;
     const uint32_t bs = pick_block(e);
-    const size_t lds = e->rom_bytes + (size_t)bs * fa::STATE_BYTES;
+    const size_t lds = lds_bytes(e, bs);
     if (lds > e->max_lds) return fail(FA_EINVAL, "LDS budget exceeded for this block size");
     const uint32_t grid = (uint32_t)((e->n + bs - 1) / bs);
     if (grid == 0) return FA_OK;
@@ -397,13 +399,16 @@ int fa_create(const uint8_t *rom, size_t rom_len, size_t n, fa_env **out) {
                         (int)e->max_lds);
     hipFuncSetAttribute((const void *)fa::fa_frame_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                         (int)e->max_lds);
-    if (hipMalloc(&e->d_rom, e->rom_bytes) != hipSuccess ||
+    if (hipMalloc(&e->d_desc, 2048) != hipSuccess || hipMalloc(&e->d_rom, e->rom_bytes) != hipSuccess ||
         hipMalloc(&e->d_states, e->n_pad * fa::STATE_BYTES) != hipSuccess ||
         hipMalloc(&e->d_cpu, e->n_pad * 8) != hipSuccess || hipMalloc(&e->d_fcyc, e->n_pad * 8) != hipSuccess ||
         hipMalloc(&e->d_instr, e->n_pad * 8) != hipSuccess || hipMalloc(&e->d_status, e->n_pad * 4) != hipSuccess ||
         hipMalloc(&e->d_obs, e->n_pad * 128) != hipSuccess || hipMalloc(&e->d_fs, e->n_pad) != hipSuccess)
         return cleanup(fail(FA_ENOMEM, "device allocation failed"));
-    if (hipMemcpy(e->d_rom, e->rom_image.data(), e->rom_bytes, hipMemcpyHostToDevice) != hipSuccess)
+    uint32_t desc[512];
+    fa::build_descriptors(desc);
+    if (hipMemcpy(e->d_rom, e->rom_image.data(), e->rom_bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(e->d_desc, desc, sizeof(desc), hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(FA_EHIP, "ROM upload failed"));
     (void)rc;
     // Fresh handles hold the power-on state until reset() is called.
@@ -430,7 +435,7 @@ void fa_destroy(fa_env *e) {
         hipEventDestroy(p.first);
         hipEventDestroy(p.second);
     }
-    for (void *p : {(void *)e->d_rom, (void *)e->d_states, (void *)e->d_actions, (void *)e->d_fs, (void *)e->d_obs,
+    for (void *p : {(void *)e->d_desc, (void *)e->d_rom, (void *)e->d_states, (void *)e->d_actions, (void *)e->d_fs, (void *)e->d_obs,
                     (void *)e->d_cpu, (void *)e->d_fcyc, (void *)e->d_instr, (void *)e->d_status})
         if (p) hipFree(p);
     if (e->stream) hipStreamDestroy(e->stream);
@@ -605,9 +610,15 @@ int fa_set_block_size(fa_env *e, uint32_t block) {
     if (!e) return fail(FA_EINVAL, "NULL handle");
     if (block != 0 && (block % 64 != 0 || block > 256 || (kPadEnvs % block) != 0))
         return fail(FA_EINVAL, "block must be 0, 64, 128 or 256");
-    if (block && e->rom_bytes + (size_t)block * fa::STATE_BYTES > e->max_lds)
+    if (block && lds_bytes(e, block) > e->max_lds)
         return fail(FA_EINVAL, "block too large for the LDS budget with this ROM");
     e->block = block;
+    return FA_OK;
+}
+
+int fa_get_max_lds(const fa_env *e, size_t *bytes) {
+    if (!e || !bytes) return fail(FA_EINVAL, "NULL argument");
+    *bytes = e->max_lds;
     return FA_OK;
 }
 

@@ -107,6 +107,7 @@ def load_library():
         "fa_kernel_time": (C.c_int, [vp, C.POINTER(C.c_double), u64p, C.c_int]),
         "fa_set_block_size": (C.c_int, [vp, C.c_uint32]),
         "fa_get_block_size": (C.c_int, [vp, u32p]),
+        "fa_get_max_lds": (C.c_int, [vp, C.POINTER(C.c_size_t)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -276,6 +277,11 @@ class BatchAtariGpu:
 
     def set_block_size(self, block: int):
         _check(load_library().fa_set_block_size(self._h, int(block)))
+
+    def max_lds(self) -> int:
+        b = C.c_size_t()
+        _check(load_library().fa_get_max_lds(self._h, C.byref(b)))
+        return b.value
 
     def block_size(self) -> int:
         b = C.c_uint32()

@@ -145,6 +145,8 @@ int fa_kernel_time(fa_env *env, double *total_ms, uint64_t *launches, int reset)
 /* Launch shape: workgroup size (multiple of 64, <= 1024; 0 = automatic). */
 int fa_set_block_size(fa_env *env, uint32_t block);
 int fa_get_block_size(const fa_env *env, uint32_t *block);
+/* LDS bytes available to one workgroup on the handle's device. */
+int fa_get_max_lds(const fa_env *env, size_t *bytes);
 
 #ifdef __cplusplus
 }
