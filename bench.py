@@ -71,6 +71,7 @@ def main():
     ap.add_argument("--k", type=int, default=16, help="frames per launch of the batched path")
     ap.add_argument("--envs", type=int, default=N_ENVS)
     ap.add_argument("--sorted", action="store_true")
+    ap.add_argument("--block", type=int, default=0, help="workgroup size (0: automatic)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="f8",
@@ -104,6 +105,8 @@ def main():
 
     gpu = fastatari.BatchAtariGpu(rom, n)
     gpu.set_sorted(args.sorted)
+    if args.block:
+        gpu.set_block_size(args.block)
     gpu.reset()
     gpu.set_timing(True)
 
