@@ -207,6 +207,35 @@ constexpr void describe(const Op &o, uint32_t &d0, uint32_t &d1) {
 
 }  // namespace detail
 
+// Descriptor of one opcode, usable at compile time: {d0, d1} packed as
+// d0 | d1 << 32 (CLS_ILLEGAL with length 1 for undocumented opcodes).
+constexpr uint64_t descriptor_of(uint32_t code) {
+    for (const detail::Op &o : detail::kOps) {
+        if (o.code == code) {
+            uint32_t d0 = 0, d1 = 0;
+            detail::describe(o, d0, d1);
+            return (uint64_t)d0 | ((uint64_t)d1 << 32);
+        }
+    }
+    return (uint64_t)((CLS_ILLEGAL << D_CLS_SHIFT) | (1u << D_LEN_SHIFT));
+}
+
+// The 151 documented opcodes, for code generation over the instruction set.
+#define FA_FOR_EACH_OPCODE(X)                                                                              \
+    X(0x00) X(0x01) X(0x05) X(0x06) X(0x08) X(0x09) X(0x0A) X(0x0D) X(0x0E) X(0x10) X(0x11) X(0x15)       \
+    X(0x16) X(0x18) X(0x19) X(0x1D) X(0x1E) X(0x20) X(0x21) X(0x24) X(0x25) X(0x26) X(0x28) X(0x29)       \
+    X(0x2A) X(0x2C) X(0x2D) X(0x2E) X(0x30) X(0x31) X(0x35) X(0x36) X(0x38) X(0x39) X(0x3D) X(0x3E)       \
+    X(0x40) X(0x41) X(0x45) X(0x46) X(0x48) X(0x49) X(0x4A) X(0x4C) X(0x4D) X(0x4E) X(0x50) X(0x51)       \
+    X(0x55) X(0x56) X(0x58) X(0x59) X(0x5D) X(0x5E) X(0x60) X(0x61) X(0x65) X(0x66) X(0x68) X(0x69)       \
+    X(0x6A) X(0x6C) X(0x6D) X(0x6E) X(0x70) X(0x71) X(0x75) X(0x76) X(0x78) X(0x79) X(0x7D) X(0x7E)       \
+    X(0x81) X(0x84) X(0x85) X(0x86) X(0x88) X(0x8A) X(0x8C) X(0x8D) X(0x8E) X(0x90) X(0x91) X(0x94)       \
+    X(0x95) X(0x96) X(0x98) X(0x99) X(0x9A) X(0x9D) X(0xA0) X(0xA1) X(0xA2) X(0xA4) X(0xA5) X(0xA6)       \
+    X(0xA8) X(0xA9) X(0xAA) X(0xAC) X(0xAD) X(0xAE) X(0xB0) X(0xB1) X(0xB4) X(0xB5) X(0xB6) X(0xB8)       \
+    X(0xB9) X(0xBA) X(0xBC) X(0xBD) X(0xBE) X(0xC0) X(0xC1) X(0xC4) X(0xC5) X(0xC6) X(0xC8) X(0xC9)       \
+    X(0xCA) X(0xCC) X(0xCD) X(0xCE) X(0xD0) X(0xD1) X(0xD5) X(0xD6) X(0xD8) X(0xD9) X(0xDD) X(0xDE)       \
+    X(0xE0) X(0xE1) X(0xE4) X(0xE5) X(0xE6) X(0xE8) X(0xE9) X(0xEA) X(0xEC) X(0xED) X(0xEE) X(0xF0)       \
+    X(0xF1) X(0xF5) X(0xF6) X(0xF8) X(0xF9) X(0xFD) X(0xFE)
+
 // Fills table[2 * op] = d0, table[2 * op + 1] = d1 for all 256 opcodes;
 // undocumented opcodes are CLS_ILLEGAL with length 1.
 inline void build_descriptors(uint32_t *table) {

@@ -554,28 +554,190 @@ struct Emu {
         return cyc;
     }
 
-    // Fetch, advance PC and execute one instruction.  Returns the cycle
-    // count, or 0 for an illegal opcode.
-    //
-    // Fast path: the instruction is fetched from the ROM window away from the
-    // hotspots, is an ALU / load / store / RMW / register / flag instruction
-    // or a branch, and every data access it makes is RAM or hotspot-free ROM
-    // (no TIA / RIOT register, no decimal-mode arithmetic).  Such an
-    // instruction has no bus side effect other than its RAM write, so it is
-    // committed with straight-line predicated code.  Every other lane takes
-    // the general path (execute()), which replays the reference's bus
-    // accesses one by one.
-    __device__ __forceinline__ uint32_t exec() {
-        const uint32_t pc0 = pc;
-        const bool fz = (pc0 & 0x1000u) && ((pc0 & 0xFFFu) < 0xFF0u);
-        const uint32_t o = ROM0 + (bank << 12) + (pc0 & 0xFFFu);
-        const uint32_t op = ld8(o), b1 = ld8(o + 1u), b2 = ld8(o + 2u);
-        const uint32_t d0 = ld32(DESC0 + op * 8u), d1 = ld32(DESC0 + op * 8u + 4u);
+    // ------------------------------------------------------------ fast paths
+    // Side effects of a discarded read at a13 (13-bit address): a hotspot or INTIM.
+    __device__ __forceinline__ bool touch_has_effect(uint32_t a13) const {
+        return ((a13 & 0x1000u) && ((a13 & 0xFFFu) - hot_lo) < hot_n) || (a13 & 0x1287u) == 0x284u;
+    }
+
+    // Convergent fast path: every active lane executes the same opcode, whose
+    // descriptor (u0, u1) is wave-uniform (SGPRs), so each descriptor test
+    // below is a scalar branch and only the work of this opcode's stages is
+    // issued.  TIA / RIOT accesses are made in place (exact, in order).
+    // Returns false, having changed nothing, when the lane needs the general
+    // path: a hotspot or INTIM reached by a phantom read, a bankswitch data
+    // access, a zero-page pointer or stack byte in the TIA, JMP (ind) or an
+    // illegal opcode.
+    __device__ __forceinline__ bool fast_uniform(uint32_t u0, uint32_t u1, uint32_t b1, uint32_t b2, uint32_t pc0,
+                                                 uint32_t &cyc) {
+        const uint32_t cls = u0 & 3u;
+        const uint32_t pcn = (pc0 + ((u0 >> D_LEN_SHIFT) & 3u)) & 0xFFFFu;
+        cyc = (u0 >> D_CYC_SHIFT) & 7u;
+        if (cls == CLS_BRANCH) {
+            const uint32_t fm = (u1 >> D1_FMASK_SHIFT) & 0xFFu;
+            const bool taken = ((p & fm) != 0u) == ((u1 >> D1_FVAL_SHIFT) != 0u);
+            const uint32_t np = (pcn + b1 - ((b1 & 0x80u) << 1)) & 0xFFFFu;
+            const bool bcross = ((np ^ pcn) & 0xFF00u) != 0u;
+            if (taken && bcross && touch_has_effect(((pcn & 0xFF00u) | (np & 0xFFu)) & 0x1FFFu)) return false;
+            pc = pick(taken, np, pcn);
+            cyc += (uint32_t)taken + (uint32_t)(taken && bcross);
+            return true;
+        }
+        if (cls == CLS_SPECIAL) {
+            const uint32_t spec = (u1 >> D1_SPEC_SHIFT) & 15u;
+            const uint32_t s1 = (sp + 1u) & 0xFFu, s2 = (sp + 2u) & 0xFFu, s3 = (sp + 3u) & 0xFFu;
+            const uint32_t m1 = (sp - 1u) & 0xFFu, m2 = (sp - 2u) & 0xFFu;
+            switch (spec) {
+            case SP_JMP: pc = b1 | (b2 << 8); return true;
+            case SP_JSR: {
+                if (!(sp & m1 & 0x80u)) return false;
+                const uint32_t ret = (pcn - 1u) & 0xFFFFu;
+                st8(ram_off(sp), ret >> 8);
+                st8(ram_off(m1), ret & 0xFFu);
+                sp = m2;
+                pc = b1 | (b2 << 8);
+                return true;
+            }
+            case SP_RTS: {
+                if (!(s1 & s2 & 0x80u)) return false;
+                const uint32_t ad = ld8(ram_off(s1)) | (ld8(ram_off(s2)) << 8);
+                if (touch_has_effect(ad & 0x1FFFu)) return false;
+                sp = s2;
+                pc = (ad + 1u) & 0xFFFFu;
+                return true;
+            }
+            case SP_RTI: {
+                if (!(s1 & s2 & s3 & 0x80u)) return false;
+                p = ld8(ram_off(s1)) & ~(FB | FU);
+                pc = ld8(ram_off(s2)) | (ld8(ram_off(s3)) << 8);
+                sp = s3;
+                return true;
+            }
+            case SP_BRK: {
+                if (!(sp & m1 & m2 & 0x80u)) return false;
+                const uint32_t ret = (pcn + 1u) & 0xFFFFu;
+                st8(ram_off(sp), ret >> 8);
+                st8(ram_off(m1), ret & 0xFFu);
+                st8(ram_off(m2), p | FB | FU);
+                sp = (sp - 3u) & 0xFFu;
+                p |= FI;
+                const uint32_t vb = ROM0 + (bank << 12);     // $FFFE/$FFFF are never hotspots
+                pc = ld8(vb + 0xFFEu) | (ld8(vb + 0xFFFu) << 8);
+                return true;
+            }
+            case SP_PHA: case SP_PHP:
+                if (!(sp & 0x80u)) return false;
+                st8(ram_off(sp), spec == SP_PHA ? a : (p | FB | FU));
+                sp = m1;
+                pc = pcn;
+                return true;
+            case SP_PLA: case SP_PLP: {
+                if (!(s1 & 0x80u)) return false;
+                const uint32_t v = ld8(ram_off(s1));
+                sp = s1;
+                if (spec == SP_PLA) { a = v; set_zn(v); } else { p = v & ~(FB | FU); }
+                pc = pcn;
+                return true;
+            }
+            default: return false;                              // JMP (ind)
+            }
+        }
+        if (cls == CLS_ILLEGAL) return false;
+
+        // ---- compute class
+        const uint32_t srcsel = (u1 >> D1_SRC_SHIFT) & 7u;
+        const uint32_t s = srcsel == R_A ? a : srcsel == R_X ? x : srcsel == R_Y ? y : sp;
+        uint32_t v = (u0 & D_IMM) ? b1 : s;
+        uint32_t a13 = 0;
+        bool io = false, ramw = false;
+        if (u0 & D_MEM) {
+            const uint32_t ind = (u0 >> D_IND_SHIFT) & 3u;
+            const uint32_t ixsel = (u0 >> D_IDX_SHIFT) & 3u;
+            uint32_t base, ea;
+            if (ind) {
+                const uint32_t zp = ind == 1u ? ((b1 + x) & 0xFFu) : b1;
+                const uint32_t zp1 = (zp + 1u) & 0xFFu;
+                if (!(zp & zp1 & 0x80u)) return false;          // pointer byte in the TIA
+                base = ld8(ram_off(zp)) | (ld8(ram_off(zp1)) << 8);
+                ea = ind == 2u ? (base + y) & 0xFFFFu : base;
+            } else if (u0 & D_ZP) {
+                base = b1;
+                ea = ixsel == 0u ? b1 : ((b1 + (ixsel == 1u ? x : y)) & 0xFFu);
+            } else {
+                base = b1 | (b2 << 8);
+                ea = ixsel == 0u ? base : ((base + (ixsel == 1u ? x : y)) & 0xFFFFu);
+            }
+            a13 = ea & 0x1FFFu;
+            if (u0 & (D_DUMMY_ALWAYS | D_DUMMY_CROSS)) {
+                const bool cross = ((base ^ ea) & 0xFF00u) != 0u;
+                const bool fix = (u0 & D_DUMMY_ALWAYS) || cross;
+                if (fix && touch_has_effect(((base & 0xFF00u) | (ea & 0xFFu)) & 0x1FFFu)) return false;
+                if (u0 & D_DUMMY_CROSS) cyc += (uint32_t)cross;
+            }
+            const bool rom = (a13 & 0x1000u) != 0u;
+            if (rom && ((a13 & 0xFFFu) - hot_lo) < hot_n) return false;   // bankswitch access
+            if (u0 & D_RD) {
+                const bool ram = (a13 & 0x1E80u) == 0x0080u;
+                v = ld8(rom ? ROM0 + (bank << 12) + (a13 & 0xFFFu) : ram_off(a13));
+                io = !rom && !ram;
+                if (io) v = io_read(a13);
+            }
+            if (u0 & D_WR) { ramw = (a13 & 0x1280u) == 0x0080u; io = !rom && !ramw; }
+        }
+        // ---- compute (scalar branches on the kind)
+        const uint32_t kind = (u0 >> D_KIND_SHIFT) & 7u;
+        const uint32_t p0 = p, a0 = a;
+        uint32_t r = s;
+        uint32_t q = p;
+        if (kind == K_LOGIC) {
+            r = u0 & D_M_LD ? v : (u0 & D_M_XOR ? (u0 & D_M_AND ? (a | v) : (a ^ v)) : (a & v));
+        } else if (kind == K_ARITH) {
+            const uint32_t vv = (u0 & D_INV) ? (v ^ 0xFFu) : v;
+            const uint32_t sum = s + vv + ((u0 & D_CIN1) ? 1u : (p & FC));
+            r = sum & 0xFFu;
+            q = (q & ~FC) | ((sum >> 8) & 1u);
+            if (u0 & D_SETV) q = (q & ~FV) | ((~(s ^ vv) & (s ^ r) & 0x80u) >> 1);
+        } else if (kind == K_RMW) {
+            const uint32_t rop = (u1 >> D1_RMW_SHIFT) & 7u;
+            if (rop < 2u) { r = ((v << 1) | (rop == 1u ? (p & FC) : 0u)) & 0xFFu; q = (q & ~FC) | (v >> 7); }
+            else if (rop < 4u) { r = (v >> 1) | (rop == 3u ? ((p & FC) << 7) : 0u); q = (q & ~FC) | (v & 1u); }
+            else r = (v + (rop == 5u ? 1u : 0xFFu)) & 0xFFu;
+        } else if (kind == K_MOVE) {
+            const uint32_t dl = (u1 >> D1_DELTA_SHIFT) & 3u;
+            r = (s + (dl == 1u ? 1u : (dl == 2u ? 0xFFu : 0u))) & 0xFFu;
+        } else if (kind == K_BIT) {
+            q = (q & ~(FN | FV | FZ)) | (v & 0xC0u) | ((a & v) == 0u ? FZ : 0u);
+        } else if (kind == K_FLAGS) {
+            const uint32_t fm = (u1 >> D1_FMASK_SHIFT) & 0xFFu;
+            q = (q & ~fm) | (u1 >> D1_FVAL_SHIFT);
+        }
+        if (u0 & D_SETNZ) q = (q & ~(FN | FZ)) | (r & 0x80u) | (r == 0u ? FZ : 0u);
+        p = q;
+        const uint32_t dst = (u1 >> D1_DST_SHIFT) & 7u;
+        if (dst == R_A) a = r;
+        else if (dst == R_X) x = r;
+        else if (dst == R_Y) y = r;
+        else if (dst == R_SP) sp = r;
+        if ((u0 & D_DEC) && (p0 & FD)) decimal((u0 & D_INV) != 0u, a0, v, p0);
+        if (u0 & D_WR) {
+            st8(ramw ? ram_off(a13) : junk, r);                 // ROM / I/O targets hit the discard byte
+            if (io) {
+                if (u0 & D_RMWMEM) io_write(a13, v);
+                io_write(a13, r);
+            }
+        }
+        pc = pcn;
+        return true;
+    }
+
+    // Divergent fast path (per-lane descriptors, predicated selects); RAM /
+    // ROM accesses only.  Returns false, having changed nothing, otherwise.
+    __device__ __forceinline__ bool fast_divergent(uint32_t d0, uint32_t d1, uint32_t b1, uint32_t b2, uint32_t pc0,
+                                                   uint32_t &cyc_out) {
         const uint32_t cls = d0 & 3u;
         const uint32_t pcn = (pc0 + ((d0 >> D_LEN_SHIFT) & 3u)) & 0xFFFFu;
-
         // ---- side-effect-free decode: effective address and eligibility
-        uint32_t ok = (uint32_t)fz & (uint32_t)(cls <= CLS_BRANCH) & ~(uint32_t)(((d0 & D_DEC) != 0u) & ((p & FD) != 0u));
+        uint32_t ok = (uint32_t)(cls <= CLS_BRANCH) & ~(uint32_t)(((d0 & D_DEC) != 0u) & ((p & FD) != 0u));
         uint32_t base = pick(d0 & D_ZP, b1, b1 | (b2 << 8));
         uint32_t ixsel = (d0 >> D_IDX_SHIFT) & 3u;
         const uint32_t ind = (d0 >> D_IND_SHIFT) & 3u;
@@ -610,8 +772,8 @@ struct Emu {
                                (uint32_t)((pp13 & 0x1287u) == 0x284u);
         ok &= mem_ok & ~(taken & bcross & pp_fx) & 1u;
 
-        uint32_t cyc;
-        if (ok) {
+        if (!ok) return false;
+        {
             const uint32_t kind = (d0 >> D_KIND_SHIFT) & 7u;
             const uint32_t s = reg((d1 >> D1_SRC_SHIFT) & 7u);
             const uint32_t mval = ld8(pick(is_rom, ROM0 + (bank << 12) + (a13 & 0xFFFu), ram_off(a13)));
@@ -652,8 +814,37 @@ struct Emu {
             // the phantom write of an RMW to RAM rewrites the same byte: only the final write matters
             st8(pick(d0 & D_WR, ram_off(a13), junk), r);
             pc = pick(taken, np, pcn);
-            cyc = ((d0 >> D_CYC_SHIFT) & 7u) + dcross + (taken & 1u) + (taken & bcross & 1u);
-        } else {
+            cyc_out = ((d0 >> D_CYC_SHIFT) & 7u) + dcross + (taken & 1u) + (taken & bcross & 1u);
+        }
+        return true;
+    }
+
+    // Fetch, advance PC and execute one instruction.  Returns the cycle
+    // count, or 0 for an illegal opcode.  Lanes whose PC is inside the ROM
+    // window (below the hotspot / vector area) fetch from LDS without side
+    // effects; if the whole wave agrees on the opcode it takes the
+    // convergent path, otherwise the divergent one; anything either declines
+    // goes through the general path (execute()), which replays the
+    // reference's bus accesses one by one.
+    __device__ __forceinline__ uint32_t exec() {
+        const uint32_t pc0 = pc;
+        const bool fz = (pc0 & 0x1000u) && ((pc0 & 0xFFFu) < 0xFF0u);
+        const uint32_t o = ROM0 + (bank << 12) + (pc0 & 0xFFFu);
+        const uint32_t op = ld8(o), b1 = ld8(o + 1u), b2 = ld8(o + 2u);
+        const uint32_t d0 = ld32(DESC0 + op * 8u), d1 = ld32(DESC0 + op * 8u + 4u);
+        const uint32_t key = fz ? op : 0x100u;
+        const uint32_t lk = (uint32_t)__builtin_amdgcn_readfirstlane((int)key);
+        const bool convergent = __ballot(key == lk) == __ballot(1) && lk < 0x100u;
+        uint32_t cyc = 0;
+        bool done = false;
+        if (convergent) {
+            const uint32_t u0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d0);
+            const uint32_t u1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d1);
+            done = fast_uniform(u0, u1, b1, b2, pc0, cyc);
+        } else if (fz) {
+            done = fast_divergent(d0, d1, b1, b2, pc0, cyc);
+        }
+        if (!done) {
             uint32_t gb1 = b1, gb2 = b2, gd0 = d0, gd1 = d1, gop = op;
             if (!fz) fetch(gb1, gb2, gd0, gd1, gop);
             pc = (pc0 + ((gd0 >> D_LEN_SHIFT) & 3u)) & 0xFFFFu;

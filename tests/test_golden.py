@@ -54,4 +54,6 @@ def test_gpu_matches_golden(name):
     for _ in range(GOLD["frames"]):
         gpu.step(acts.next())
         got.append(dig())
-    assert got == GOLD["cases"][name]["state_sha256"]
+    want = GOLD["cases"][name]["state_sha256"]
+    bad = [i for i, (g, w) in enumerate(zip(got, want)) if g != w]
+    assert not bad, f"{name}: first differing frame {bad[0]} (0 = after reset)"
