@@ -730,122 +730,49 @@ struct Emu {
         return true;
     }
 
-    // Divergent fast path (per-lane descriptors, predicated selects); RAM /
-    // ROM accesses only.  Returns false, having changed nothing, otherwise.
-    __device__ __forceinline__ bool fast_divergent(uint32_t d0, uint32_t d1, uint32_t b1, uint32_t b2, uint32_t pc0,
-                                                   uint32_t &cyc_out) {
-        const uint32_t cls = d0 & 3u;
-        const uint32_t pcn = (pc0 + ((d0 >> D_LEN_SHIFT) & 3u)) & 0xFFFFu;
-        // ---- side-effect-free decode: effective address and eligibility
-        uint32_t ok = (uint32_t)(cls <= CLS_BRANCH) & ~(uint32_t)(((d0 & D_DEC) != 0u) & ((p & FD) != 0u));
-        uint32_t base = pick(d0 & D_ZP, b1, b1 | (b2 << 8));
-        uint32_t ixsel = (d0 >> D_IDX_SHIFT) & 3u;
-        const uint32_t ind = (d0 >> D_IND_SHIFT) & 3u;
-        if (ind) {
-            const uint32_t zp = pick(ind == 1u, (b1 + x) & 0xFFu, b1);
-            const uint32_t zp1 = (zp + 1u) & 0xFFu;
-            ok &= (uint32_t)((zp & zp1 & 0x80u) != 0u);      // pointer in RAM, not in the TIA
-            base = ld8(ram_off(zp)) | (ld8(ram_off(zp1)) << 8);
-            ixsel = (ind & 2u);                                 // (zp),Y indexes by Y; (zp,X) not at all
-        }
-        const uint32_t idx = (x & mk(ixsel == 1u)) | (y & mk(ixsel == 2u));
-        const uint32_t ea = (base + idx) & pick(d0 & D_ZP, 0xFFu, 0xFFFFu);
-        const uint32_t a13 = ea & 0x1FFFu;
-        const uint32_t is_rom = (uint32_t)((a13 & 0x1000u) != 0u);
-        const uint32_t hot = is_rom & (uint32_t)(((a13 & 0xFFFu) - hot_lo) < hot_n);
-        const uint32_t cross = (uint32_t)(((base ^ ea) & 0xFF00u) != 0u);
-        const uint32_t dcross = (uint32_t)((d0 & D_DUMMY_CROSS) != 0u) & cross;
-        const uint32_t fix = (uint32_t)((d0 & D_DUMMY_ALWAYS) != 0u) | dcross;
-        const uint32_t f13 = ((base & 0xFF00u) | (ea & 0xFFu)) & 0x1FFFu;
-        const uint32_t fix_fx = ((uint32_t)((f13 & 0x1000u) != 0u) & (uint32_t)(((f13 & 0xFFFu) - hot_lo) < hot_n)) |
-                                (uint32_t)((f13 & 0x1287u) == 0x284u);
-        const uint32_t rd_ok = (uint32_t)((d0 & D_RD) == 0u) | (uint32_t)((a13 & 0x1E80u) == 0x80u) | (is_rom & ~hot & 1u);
-        const uint32_t wr_ok = (uint32_t)((d0 & D_WR) == 0u) | (uint32_t)((a13 & 0x1280u) == 0x80u);
-        const uint32_t mem_ok = (uint32_t)((d0 & D_MEM) == 0u) | (~(fix & fix_fx) & rd_ok & wr_ok & 1u);
-        // branch target and the partial-PC phantom read of a page-crossing branch
-        const uint32_t fm = (d1 >> D1_FMASK_SHIFT) & 0xFFu;
-        const uint32_t taken = (uint32_t)(cls == CLS_BRANCH) & (uint32_t)(((p & fm) != 0u) == ((d1 >> D1_FVAL_SHIFT) != 0u));
-        const uint32_t np = (pcn + b1 - ((b1 & 0x80u) << 1)) & 0xFFFFu;
-        const uint32_t bcross = (uint32_t)(((np ^ pcn) & 0xFF00u) != 0u);
-        const uint32_t pp13 = ((pcn & 0xFF00u) | (np & 0xFFu)) & 0x1FFFu;
-        const uint32_t pp_fx = ((uint32_t)((pp13 & 0x1000u) != 0u) & (uint32_t)(((pp13 & 0xFFFu) - hot_lo) < hot_n)) |
-                               (uint32_t)((pp13 & 0x1287u) == 0x284u);
-        ok &= mem_ok & ~(taken & bcross & pp_fx) & 1u;
-
-        if (!ok) return false;
-        {
-            const uint32_t kind = (d0 >> D_KIND_SHIFT) & 7u;
-            const uint32_t s = reg((d1 >> D1_SRC_SHIFT) & 7u);
-            const uint32_t mval = ld8(pick(is_rom, ROM0 + (bank << 12) + (a13 & 0xFFFu), ram_off(a13)));
-            const uint32_t v = pick(d0 & D_IMM, b1, pick(d0 & D_RD, mval, s));
-            const uint32_t lg = ((a & v) & mk(d0 & D_M_AND)) | ((a ^ v) & mk(d0 & D_M_XOR)) | (v & mk(d0 & D_M_LD));
-            const uint32_t vv = v ^ (0xFFu & mk(d0 & D_INV));
-            const uint32_t cin = p & FC;
-            const uint32_t sum = s + vv + pick(d0 & D_CIN1, 1u, cin);
-            const uint32_t ar = sum & 0xFFu;
-            const uint32_t rop = (d1 >> D1_RMW_SHIFT) & 7u;
-            const uint32_t shl = ((v << 1) | (cin & mk(rop == 1u))) & 0xFFu;
-            const uint32_t shr = (v >> 1) | ((cin << 7) & mk(rop == 3u));
-            const uint32_t idc = (v + pick(rop == 5u, 1u, 0xFFu)) & 0xFFu;
-            const uint32_t rr = (shl & mk(rop < 2u)) | (shr & mk((rop >> 1) == 1u)) | (idc & mk(rop >= 4u));
-            const uint32_t dl = (d1 >> D1_DELTA_SHIFT) & 3u;
-            const uint32_t mv = (s + (1u & mk(dl == 1u)) + (0xFFu & mk(dl == 2u))) & 0xFFu;
-            const uint32_t r = (lg & mk(kind == K_LOGIC)) | (ar & mk(kind == K_ARITH)) | (rr & mk(kind == K_RMW)) |
-                               (mv & mk(kind == K_MOVE)) | (s & mk(kind == K_STORE));
-            const uint32_t c_new = pick(kind == K_ARITH, (sum >> 8) & 1u, pick(rop < 2u, v >> 7, v & 1u));
-            const uint32_t v_new = (~(s ^ vv) & (s ^ ar) & 0x80u) >> 1;
-            const uint32_t m_nz = (FN | FZ) & mk(d0 & D_SETNZ);
-            const uint32_t m_c = FC & mk(d0 & D_SETC);
-            const uint32_t m_v = FV & mk(d0 & D_SETV);
-            const uint32_t m_bit = (FN | FV | FZ) & mk(kind == K_BIT);
-            const uint32_t m_fl = fm & mk(kind == K_FLAGS);
-            uint32_t q = p;
-            q = (q & ~m_nz) | (((r & 0x80u) | (FZ & mk(r == 0u))) & m_nz);
-            q = (q & ~m_c) | (c_new & m_c);
-            q = (q & ~m_v) | (v_new & m_v);
-            q = (q & ~m_bit) | (((v & 0xC0u) | (FZ & mk((a & v) == 0u))) & m_bit);
-            q = (q & ~m_fl) | ((d1 >> D1_FVAL_SHIFT) & m_fl);
-            p = q;
-            const uint32_t dst = (d1 >> D1_DST_SHIFT) & 7u;
-            a = pick(dst == R_A, r, a);
-            x = pick(dst == R_X, r, x);
-            y = pick(dst == R_Y, r, y);
-            sp = pick(dst == R_SP, r, sp);
-            // the phantom write of an RMW to RAM rewrites the same byte: only the final write matters
-            st8(pick(d0 & D_WR, ram_off(a13), junk), r);
-            pc = pick(taken, np, pcn);
-            cyc_out = ((d0 >> D_CYC_SHIFT) & 7u) + dcross + (taken & 1u) + (taken & bcross & 1u);
-        }
-        return true;
-    }
-
     // Fetch, advance PC and execute one instruction.  Returns the cycle
     // count, or 0 for an illegal opcode.  Lanes whose PC is inside the ROM
     // window (below the hotspot / vector area) fetch from LDS without side
-    // effects; if the whole wave agrees on the opcode it takes the
-    // convergent path, otherwise the divergent one; anything either declines
+    // effects.  When the active lanes agree on the opcode, the wave takes
+    // the convergent path: a scalar dispatch on the opcode into the
+    // fast path specialised with that opcode's compile-time descriptor.
+    // With GROUP, lanes are first split into groups of equal opcode (64-bit
+    // ballot + readfirstlane) and every group takes the convergent path.
+    // Anything the fast path declines -- and divergent waves without GROUP --
     // goes through the general path (execute()), which replays the
     // reference's bus accesses one by one.
+    template <bool GROUP>
     __device__ __forceinline__ uint32_t exec() {
         const uint32_t pc0 = pc;
         const bool fz = (pc0 & 0x1000u) && ((pc0 & 0xFFFu) < 0xFF0u);
         const uint32_t o = ROM0 + (bank << 12) + (pc0 & 0xFFFu);
         const uint32_t op = ld8(o), b1 = ld8(o + 1u), b2 = ld8(o + 2u);
-        const uint32_t d0 = ld32(DESC0 + op * 8u), d1 = ld32(DESC0 + op * 8u + 4u);
         const uint32_t key = fz ? op : 0x100u;
-        const uint32_t lk = (uint32_t)__builtin_amdgcn_readfirstlane((int)key);
-        const bool convergent = __ballot(key == lk) == __ballot(1) && lk < 0x100u;
         uint32_t cyc = 0;
         bool done = false;
-        if (convergent) {
-            const uint32_t u0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d0);
-            const uint32_t u1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d1);
-            done = fast_uniform(u0, u1, b1, b2, pc0, cyc);
-        } else if (fz) {
-            done = fast_divergent(d0, d1, b1, b2, pc0, cyc);
-        }
+        uint64_t rem = __ballot(1);
+        do {
+            const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)__builtin_ctzll(rem));
+            const bool mine = key == lk;
+            const uint64_t grp = __ballot(mine);
+            if ((GROUP || grp == rem) && mine && lk < 0x100u) {
+                switch (lk) {
+#define FA_CASE(OPC)                                                                          \
+                case OPC: {                                                                   \
+                    constexpr uint64_t D = descriptor_of(OPC);                                \
+                    done = fast_uniform((uint32_t)D, (uint32_t)(D >> 32), b1, b2, pc0, cyc);  \
+                    break;                                                                    \
+                }
+                    FA_FOR_EACH_OPCODE(FA_CASE)
+#undef FA_CASE
+                default: break;
+                }
+            }
+            rem &= ~grp;
+        } while (GROUP && rem);
         if (!done) {
-            uint32_t gb1 = b1, gb2 = b2, gd0 = d0, gd1 = d1, gop = op;
+            uint32_t gb1 = b1, gb2 = b2, gop = op;
+            uint32_t gd0 = ld32(DESC0 + op * 8u), gd1 = ld32(DESC0 + op * 8u + 4u);
             if (!fz) fetch(gb1, gb2, gd0, gd1, gop);
             pc = (pc0 + ((gd0 >> D_LEN_SHIFT) & 3u)) & 0xFFFFu;
             cyc = execute(gb1, gb2, gd0, gd1);

@@ -1,11 +1,13 @@
 // fa_lib.hip -- frame-step kernels and the C ABI (include/fastatari.h).
 //
 // Kernels (one lane per env, workgroups of `block` lanes):
-//   fa_frame_kernel<false>  per-lane interpreter (the plain drop-in path)
-//   fa_frame_kernel<true>   wave64 opcode grouping: lanes are served one
-//                           distinct opcode at a time, the leader's opcode is
-//                           read into an SGPR and dispatched with scalar
-//                           branches, so each handler runs converged
+//   fa_frame_kernel<false>  the plain drop-in path: convergent waves take
+//                           the opcode-specialised fast path, divergent
+//                           waves the general per-lane path
+//   fa_frame_kernel<true>   wave64 opcode grouping (set_sorted): lanes are
+//                           served one distinct opcode at a time (64-bit
+//                           ballot + readlane), every group on the
+//                           opcode-specialised fast path
 //   fa_fill_kernel          power-on broadcast used by reset
 // The same frame kernel serves step (k = 1), step_multi (k frames with state
 // kept in VGPRs/LDS across frames) and frameskip (per-env frame counts).
@@ -51,27 +53,7 @@ struct StepArgs {
 // One CPU instruction for the active lanes.  Returns cycles (0 = illegal).
 template <bool SORTED>
 __device__ __forceinline__ uint32_t cpu_step(Emu &e) {
-    if (!SORTED) return e.exec();
-    // Opcode grouping: lanes are served one distinct opcode at a time; the
-    // leader's descriptor is read into SGPRs, so every select of the
-    // pipeline resolves as scalar control flow for the whole group.
-    uint32_t b1, b2, d0, d1, op;
-    e.fetch(b1, b2, d0, d1, op);
-    e.pc = (e.pc + ((d0 >> D_LEN_SHIFT) & 3u)) & 0xFFFFu;
-    uint64_t rem = __ballot(1);
-    uint32_t c = 0;
-    while (rem) {
-        const int lead = (int)__builtin_ctzll(rem);
-        const uint32_t lop = (uint32_t)__builtin_amdgcn_readlane((int)op, lead);
-        const bool mine = op == lop;
-        if (mine) {
-            const uint32_t u0 = (uint32_t)__builtin_amdgcn_readlane((int)d0, lead);
-            const uint32_t u1 = (uint32_t)__builtin_amdgcn_readlane((int)d1, lead);
-            c = e.execute(b1, b2, u0, u1);
-        }
-        rem &= ~__ballot(mine);
-    }
-    return c;
+    return e.exec<SORTED>();
 }
 
 // HeadlessAtari::run_frame (src/atari.rs:584-614).  Returns the frame's
