@@ -5,7 +5,14 @@ new / set_sorted / reset / step / step_multi / download_states / num_envs,
 over the C ABI in include/fastatari.h (libfastatari_hip.so, HIP kernels for
 gfx950).  There is no CPU fallback: constructing a BatchAtariGpu without the
 built library or without a GPU raises.
-"""
-from .env import (STATE_DTYPE, BatchAtariGpu, FastAtariError, DeviceBuffer, lib_path, load_library)
 
-__all__ = ["BatchAtariGpu", "FastAtariError", "DeviceBuffer", "STATE_DTYPE", "lib_path", "load_library"]
+`BatchMixGpu` steps several ROM classes in one hipGraph launch (per-class
+launch shapes from its autotuner); `EnvServer` serves step requests of many
+clients with continuous batching.
+"""
+from .env import (STATE_DTYPE, BatchAtariGpu, Busy, FastAtariError, DeviceBuffer, lib_path, load_library)
+from .batch import BatchMixGpu
+from .serve import EnvServer
+
+__all__ = ["BatchAtariGpu", "BatchMixGpu", "EnvServer", "Busy", "FastAtariError", "DeviceBuffer", "STATE_DTYPE",
+           "lib_path", "load_library"]

@@ -10,19 +10,21 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(ROOT, "csrc")
 LIB = os.path.join(PKG, "libfastatari_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# fa_kernels.hip holds every kernel; the others are host code over its launchers
+SOURCES = ["fa_kernels.hip", "fa_lib.hip", "fa_batch.hip", "fa_serve.hip"]
 
 FLAGS = [
     "-std=c++17", "-O3", "--offload-arch=gfx950", "-fPIC", "-shared",
     # packed FP32 instructions are disabled for every HIP translation unit
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wno-unused-result",
+    "-Wno-unused-result", "-Wno-unused-value", "-pthread",
 ]
 
 
 def build(verbose: bool = False) -> str:
-    src = os.path.join(CSRC, "fa_lib.hip")
+    srcs = [os.path.join(CSRC, f) for f in SOURCES]
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, src]
+    cmd = [HIPCC, *FLAGS, "-o", tmp, *srcs]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -28,6 +28,7 @@ FA_EINVAL = -1
 FA_EHIP = -2
 FA_ESTATE = -3
 FA_ENOMEM = -4
+FA_EBUSY = -5
 
 # AtariStateGpu (include/fastatari.h; src/cuda_env.rs:12-38), 222 bytes.
 _FIELDS = [
@@ -62,6 +63,10 @@ class FastAtariError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"fastatari error {code}: {msg}")
         self.code = code
+
+
+class Busy(FastAtariError):
+    """FA_EBUSY: over the server's memory budget, or a request in flight."""
 
 
 _lib = None
@@ -108,6 +113,37 @@ def load_library():
         "fa_set_block_size": (C.c_int, [vp, C.c_uint32]),
         "fa_get_block_size": (C.c_int, [vp, u32p]),
         "fa_get_max_lds": (C.c_int, [vp, C.POINTER(C.c_size_t)]),
+        # heterogeneous batches
+        "fa_batch_create": (C.c_int, [C.POINTER(u8p), C.POINTER(sz), C.POINTER(sz), sz, C.POINTER(vp)]),
+        "fa_batch_destroy": (None, [vp]),
+        "fa_batch_num_envs": (sz, [vp]),
+        "fa_batch_num_classes": (sz, [vp]),
+        "fa_batch_class_range": (C.c_int, [vp, sz, C.POINTER(sz), C.POINTER(sz)]),
+        "fa_batch_reset": (C.c_int, [vp, u8p]),
+        "fa_batch_step": (C.c_int, [vp, u8p, u8p, C.c_uint32, C.c_int, u8p]),
+        "fa_batch_buffers": (C.c_int, [vp, C.c_uint32, C.c_int, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]),
+        "fa_batch_launch": (C.c_int, [vp, C.c_uint32, C.c_int, C.c_int]),
+        "fa_batch_sync": (C.c_int, [vp, C.POINTER(C.c_double)]),
+        "fa_batch_download_states": (C.c_int, [vp, vp, sz]),
+        "fa_batch_upload_states": (C.c_int, [vp, vp, sz]),
+        "fa_batch_download_counters": (C.c_int, [vp, u64p, u64p, u64p, u32p]),
+        "fa_batch_set_class_config": (C.c_int, [vp, sz, C.c_uint32, C.c_int, C.c_uint32]),
+        "fa_batch_get_class_config": (C.c_int, [vp, sz, u32p, C.POINTER(C.c_int), u32p]),
+        "fa_batch_tune": (C.c_int, [vp, C.c_uint32, C.c_int, C.POINTER(C.c_double)]),
+        # server
+        "fa_server_create": (C.c_int, [C.POINTER(u8p), C.POINTER(sz), sz, sz, C.POINTER(vp)]),
+        "fa_server_destroy": (None, [vp]),
+        "fa_server_open": (C.c_int, [vp, sz, sz, u8p, C.POINTER(C.c_int)]),
+        "fa_server_close": (C.c_int, [vp, C.c_int]),
+        "fa_server_submit": (C.c_int, [vp, C.c_int, u8p, u8p, C.c_uint32]),
+        "fa_server_poll": (C.c_int, [vp, C.c_int]),
+        "fa_server_wait": (C.c_int, [vp, C.c_int, u8p, C.POINTER(C.c_double)]),
+        "fa_server_download": (C.c_int, [vp, C.c_int, vp, u64p, u64p, u32p]),
+        "fa_server_client_info": (C.c_int, [vp, C.c_int, C.POINTER(sz), C.POINTER(sz)]),
+        "fa_server_set_class_config": (C.c_int, [vp, sz, C.c_uint32, C.c_int]),
+        "fa_server_memory": (C.c_int, [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]),
+        "fa_server_samples": (C.c_int, [vp, C.POINTER(C.c_double), u32p, u32p, u32p, sz, C.POINTER(sz), u64p,
+                                        C.c_int]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -122,6 +158,8 @@ def _check(rc: int):
         msg = load_library().fa_last_error().decode(errors="replace")
         if rc == FA_EINVAL:
             raise ValueError(msg)
+        if rc == FA_EBUSY:
+            raise Busy(rc, msg)
         raise FastAtariError(rc, msg)
 
 

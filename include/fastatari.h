@@ -25,6 +25,7 @@ extern "C" {
 #define FA_EHIP -2       /* HIP runtime error */
 #define FA_ESTATE -3     /* an uploaded state is not one the emulator can reach */
 #define FA_ENOMEM -4     /* device allocation failed */
+#define FA_EBUSY -5      /* server: over the memory budget, or a request already in flight */
 
 /* ---- per-env emulation status (fa_download_counters) ---- */
 #define FA_ENV_OK 0
@@ -147,6 +148,84 @@ int fa_set_block_size(fa_env *env, uint32_t block);
 int fa_get_block_size(const fa_env *env, uint32_t *block);
 /* LDS bytes available to one workgroup on the handle's device. */
 int fa_get_max_lds(const fa_env *env, size_t *bytes);
+
+/* ---- heterogeneous batches (several ROM classes stepped together) ----
+ * No reference counterpart: the reference binds one ROM per BatchAtariGpu
+ * (src/cuda_env.rs:112-126).  Envs of class c occupy the global indices
+ * [first_c, first_c + count_c) in class order; per-env inputs and outputs use
+ * those indices.  A step launch is one hipGraph (captured per k / flags) of
+ * per-class kernel chains running concurrently. */
+typedef struct fa_batch fa_batch;
+
+int fa_batch_create(const uint8_t *const *roms, const size_t *rom_lens, const size_t *counts, size_t nclass,
+                    fa_batch **out);
+void fa_batch_destroy(fa_batch *b);
+size_t fa_batch_num_envs(const fa_batch *b);
+size_t fa_batch_num_classes(const fa_batch *b);
+int fa_batch_class_range(const fa_batch *b, size_t c, size_t *first, size_t *count);
+/* BatchAtariGpu::reset semantics for every env of every class. */
+int fa_batch_reset(fa_batch *b, uint8_t *obs_out);
+/* k steps: actions [s * N + i]; fs [s * N + i] frames each action is held
+ * (NULL = 1); obs_out: N * 128 after the last step, or k * N * 128 after
+ * every step when obs_every.  Synchronous. */
+int fa_batch_step(fa_batch *b, const uint8_t *actions, const uint8_t *fs, uint32_t k, int obs_every,
+                  uint8_t *obs_out);
+/* Device-resident stepping: the batch's own input / output buffers (sized
+ * for k steps) -- write actions / fs there, then fa_batch_launch replays the
+ * captured graph on the batch stream and fa_batch_sync waits for it and
+ * reports the launch's device time. */
+int fa_batch_buffers(fa_batch *b, uint32_t k, int obs_every, uint8_t **d_actions, uint8_t **d_fs, uint8_t **d_obs);
+int fa_batch_launch(fa_batch *b, uint32_t k, int use_fs, int obs_every);
+int fa_batch_sync(fa_batch *b, double *launch_ms);
+int fa_batch_download_states(fa_batch *b, AtariStateGpu *out, size_t n);
+int fa_batch_upload_states(fa_batch *b, const AtariStateGpu *in, size_t n);
+int fa_batch_download_counters(fa_batch *b, uint64_t *cpu_cycles, uint64_t *frame_cycles, uint64_t *instructions,
+                               uint32_t *status);
+/* Launch shape per class: block (0 = largest that fits), opcode grouping,
+ * steps per kernel launch (0 = all k in one launch). */
+int fa_batch_set_class_config(fa_batch *b, size_t c, uint32_t block, int sorted, uint32_t k_split);
+int fa_batch_get_class_config(const fa_batch *b, size_t c, uint32_t *block, int *sorted, uint32_t *k_split);
+/* Autotuner: times trial launches of each class alone on the inputs in the
+ * batch buffers (states restored after every trial) and adopts the fastest
+ * block / grouping / steps-per-launch.  trials (2 * nclass, may be NULL):
+ * per class, ms of the chosen and of the default configuration. */
+int fa_batch_tune(fa_batch *b, uint32_t k, int use_fs, double *trials);
+
+/* ---- env-step server (continuous batching) ----
+ * No reference counterpart.  Client envs stay resident in HBM; clients are
+ * admitted against budget_bytes of device memory (FA_EBUSY when they do not
+ * fit); step requests that arrive while others run are coalesced into the
+ * next launch of their ROM class.  A client has at most one request in
+ * flight.  Functions may be called from any thread. */
+typedef struct fa_server fa_server;
+
+int fa_server_create(const uint8_t *const *roms, const size_t *rom_lens, size_t nclass, size_t budget_bytes,
+                     fa_server **out);
+void fa_server_destroy(fa_server *s);
+/* New client of n_envs (1..65536) envs on ROM class cls, each started from
+ * the ROM's reset state (power-on + one frame); obs_out (n * 128, may be
+ * NULL) receives their RAM. */
+int fa_server_open(fa_server *s, size_t cls, size_t n_envs, uint8_t *obs_out, int *client_id);
+int fa_server_close(fa_server *s, int client_id);
+/* One step of every env of the client: actions[i] held for `frames` frames;
+ * restart (n bytes, may be NULL) != 0 restarts env i from the reset state
+ * first.  Returns immediately. */
+int fa_server_submit(fa_server *s, int client_id, const uint8_t *actions, const uint8_t *restart, uint32_t frames);
+/* 1 when the client's request is done, 0 otherwise (negative on error). */
+int fa_server_poll(fa_server *s, int client_id);
+/* Waits for the client's request; obs_out (n * 128) and the request's
+ * latency from submit to obs in host memory. */
+int fa_server_wait(fa_server *s, int client_id, uint8_t *obs_out, double *latency_ms);
+/* States / counters of an idle client's envs. */
+int fa_server_download(fa_server *s, int client_id, AtariStateGpu *states, uint64_t *cpu_cycles,
+                       uint64_t *frame_cycles, uint32_t *status);
+int fa_server_client_info(fa_server *s, int client_id, size_t *cls, size_t *n_envs);
+int fa_server_set_class_config(fa_server *s, size_t cls, uint32_t block, int sorted);
+int fa_server_memory(fa_server *s, size_t *used_bytes, size_t *budget_bytes, size_t *bytes_per_slot);
+/* Per completed request since the last reset: latency ms, class, envs,
+ * frames; *count = samples held; *launches = kernel launches. */
+int fa_server_samples(fa_server *s, double *ms, uint32_t *cls, uint32_t *envs, uint32_t *frames, size_t cap,
+                      size_t *count, uint64_t *launches, int reset);
 
 #ifdef __cplusplus
 }

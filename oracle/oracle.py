@@ -57,6 +57,7 @@ def lib():
             "orc_batch_set_states": (None, [vp, u8p]),
             "orc_batch_counters": (None, [vp, u64p]),
             "orc_batch_set_cycles": (None, [vp, u64p]),
+            "orc_batch_set_counters": (None, [vp, u64p]),
             "orc_batch_obs": (None, [vp, u8p]),
             "orc_batch_bench": (C.c_double, [vp, u8p, C.c_uint32, C.c_uint32]),
             "orc_flat_new": (vp, []),
@@ -186,6 +187,24 @@ class Batch:
     def set_cycles(self, c: np.ndarray):
         c = np.ascontiguousarray(c, dtype=np.uint64)
         lib().orc_batch_set_cycles(self.h, _p(c, C.c_uint64))
+
+    def set_counters(self, c: np.ndarray):
+        """[n, 4] as counters()."""
+        c = np.ascontiguousarray(c, dtype=np.uint64)
+        lib().orc_batch_set_counters(self.h, _p(c, C.c_uint64))
+
+    def restart(self, mask, snap_state: np.ndarray, snap_counters: np.ndarray):
+        """Envs where mask is set go back to a reset snapshot (state and
+        counters), as a restarted env of the server."""
+        m = np.asarray(mask, dtype=bool)
+        if not m.any():
+            return
+        st = self.states()
+        st[m] = snap_state
+        self.set_states(st)
+        cn = self.counters()
+        cn[m] = snap_counters
+        self.set_counters(cn)
 
     def obs(self) -> np.ndarray:
         out = np.zeros((self.n, 128), np.uint8)

@@ -874,6 +874,17 @@ void orc_batch_counters(void *bp, uint64_t *out)
         out[4 * i + 3] = (uint64_t)b->envs[i].error;
     }
 }
+/* Restores every counter (the inverse of orc_batch_counters). */
+void orc_batch_set_counters(void *bp, const uint64_t *in)
+{
+    orc_batch *b = (orc_batch *)bp;
+    for (uint32_t i = 0; i < b->n; i++) {
+        b->envs[i].cpu.cycles = in[4 * i + 0];
+        b->envs[i].last_frame_cycles = in[4 * i + 1];
+        b->envs[i].instructions = in[4 * i + 2];
+        b->envs[i].error = (int)in[4 * i + 3];
+    }
+}
 void orc_batch_set_cycles(void *bp, const uint64_t *cyc)
 {
     orc_batch *b = (orc_batch *)bp;

@@ -142,6 +142,7 @@ void orc_batch_states(void *b, uint8_t *out /* n x 222 */);
 void orc_batch_set_states(void *b, const uint8_t *in /* n x 222 */);
 void orc_batch_counters(void *b, uint64_t *out /* n x 4 */);
 void orc_batch_set_cycles(void *b, const uint64_t *cycles);
+void orc_batch_set_counters(void *b, const uint64_t *in /* n x 4, as orc_batch_counters */);
 void orc_batch_obs(void *b, uint8_t *out /* n x 128 */);
 void *orc_batch_env(void *b, uint32_t i);
 /* timed CPU baseline: returns wall seconds for frames x n env-frames. */

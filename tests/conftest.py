@@ -19,3 +19,10 @@ def orc():
     from oracle import oracle
     oracle.lib()
     return oracle
+
+
+@pytest.fixture(scope="session")
+def fa_mod():
+    import fastatari
+    fastatari.load_library()
+    return fastatari
