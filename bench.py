@@ -8,6 +8,12 @@ per-env SplitMix64 action streams, actions resident in HBM before timing.
 baseline is the oracle (the C restatement of the reference emulator) on a
 bounded sample of the same workload on the host's cores.
 
+The same line carries the heterogeneous batch ("mix": four ROM classes in
+one autotuned hipGraph launch of K = 8 steps with per-env frameskip) and the
+served mode ("served": 256 clients with continuous batching; p50 / p99 step
+latency beside aggregate frames/s; its bit-exactness, 1 % of envs against the
+oracle, is checked in tests/test_gpu_serve.py).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
@@ -63,6 +69,116 @@ def cpu_baseline(rom, n_sample, frames, seed, threads):
     return n_sample * frames / secs, secs
 
 
+def bench_mix(args, fastatari, W):
+    """Heterogeneous launch: four ROM classes, 1K / 40K / 12K / 3K envs, K = 8
+    steps of per-env frameskip 1-4 per hipGraph launch, autotuned shapes."""
+    seed = W.config_seed(4)
+    roms = W.mix_roms(seed)
+    counts = list(W.MIX_COUNTS)
+    mix = fastatari.BatchMixGpu(roms, counts)
+    mix.reset()
+    n, k = mix.n, 8
+    fs = np.broadcast_to(W.frameskips(seed, n), (k, n)).copy()
+    stream = W.ActionStream(seed, n)
+    t_tune = time.perf_counter()
+    tuned = mix.tune(stream.frames(k), fs)
+    t_tune = time.perf_counter() - t_tune
+    launches = args.mix_launches
+    inputs = [stream.frames(k) for _ in range(launches + 1)]
+    mix.write_inputs(inputs[0], fs)
+    mix.launch(k, use_fs=True)
+    mix.sync()                                     # warm-up (graph instantiation)
+    dev_ms = 0.0
+    t0 = time.perf_counter()
+    for L in range(launches):
+        mix.write_inputs(inputs[L + 1], fs)
+        mix.launch(k, use_fs=True)
+        dev_ms += mix.sync()
+    wall = time.perf_counter() - t0
+    frames = int(fs.astype(np.int64).sum()) * launches
+    errors = int((mix.counters()["status"] != 0).sum())
+    names = ["2KB Fixed WSYNC", "8KB F8 WSYNC", "16KB F6 no-WSYNC", "32KB F4 WSYNC"]
+    return {
+        "workload": "4 ROM classes (1K/40K/12K/3K envs), K=8 steps x per-env frameskip 1-4, one hipGraph launch",
+        "fps": round(frames / wall, 1),
+        "device_fps": round(frames / (dev_ms * 1e-3), 1),
+        "ms_per_launch": round(wall / launches * 1e3, 3),
+        "env_frames_per_launch": frames // launches,
+        "launches": launches,
+        "tune_s": round(t_tune, 2),
+        "env_errors": errors,
+        "classes": [dict(rom=nm, envs=c, **t) for nm, c, t in zip(names, counts, tuned)],
+    }
+
+
+def _pct(v, q):
+    return round(float(np.percentile(v, q)), 3) if len(v) else None
+
+
+def bench_serve(args, fastatari, W):
+    """Serving: 256 clients on the four mixed ROMs (native load driver).  A
+    closed-loop phase finds each client's saturation rate; then Poisson
+    arrivals at 0.5-2x that rate; ~1 % env restarts per step.  Reports the
+    service latency (submit -> obs in host memory) and end-to-end latency
+    (arrival -> obs) percentiles and aggregate frames/s, beside the same env
+    population stepped as one plain batch (one frame per step)."""
+    from fastatari import loadgen as LG
+    roms = W.mix_roms(W.config_seed(4))
+    specs = LG.serve_clients(W.config_seed(5))
+    counts = [sum(sp.n for sp in specs if sp.cls == c) for c in range(4)]
+    n_envs = sum(counts)
+
+    # the same population as one plain batch: every env one frame per step
+    mix = fastatari.BatchMixGpu(roms, counts)
+    mix.reset()
+    acts = W.ActionStream(W.config_seed(5), n_envs)
+    mix.write_inputs(acts.next())
+    mix.launch(1, use_fs=False)
+    mix.sync()
+    reps = 8
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        mix.launch(1, use_fs=False)
+        mix.sync()
+    plain_fps = n_envs * reps / (time.perf_counter() - t0)
+    mix.close()
+
+    srv = fastatari.EnvServer(roms, budget_bytes=64 << 30)
+    cids = [srv.open(sp.cls, sp.n)[0] for sp in specs]
+    steps = np.zeros(len(specs), np.uint64)
+    closed = srv.run_load(cids, specs, args.serve_seconds / 2, steps=steps, arrival_seed=1)
+    closed_steps = closed["steps"].copy()
+    sat = np.maximum(closed_steps, 1) / closed["wall_s"]
+    before = steps.copy()
+    opened = srv.run_load(cids, specs, args.serve_seconds, sat=sat, steps=steps, arrival_seed=5)
+    mem = srv.memory()
+    srv.close_server()
+    n_by = np.array([sp.n for sp in specs], np.float64)
+    cls = np.array([sp.cls for sp in specs])
+
+    def line(res, prev):
+        done = res["steps"].astype(np.float64) - prev
+        out = {"fps": round(float((done * n_by).sum()) / res["wall_s"], 1), "requests": int(res["requests"]),
+               "p50_ms": _pct(res["service_ms"], 50), "p99_ms": _pct(res["service_ms"], 99),
+               "e2e_p50_ms": _pct(res["e2e_ms"], 50), "e2e_p99_ms": _pct(res["e2e_ms"], 99), "per_class": {}}
+        for c in range(4):
+            m = res["cls"] == c
+            if m.any():
+                out["per_class"][str(c)] = {"p50_ms": _pct(res["service_ms"][m], 50),
+                                            "p99_ms": _pct(res["service_ms"][m], 99),
+                                            "fps": round(float((done * n_by)[cls == c].sum()) / res["wall_s"], 1)}
+        return out
+
+    return {
+        "workload": f"256 clients, {n_envs} envs resident (64-2047 per client, log-uniform) on the 4 mixed ROMs; "
+                    "Poisson arrivals at 0.5-2x saturation; 1% restarts per step; native load driver",
+        "plain_batch_fps": round(plain_fps, 1),
+        "closed_loop": line(closed, np.zeros(len(specs))),
+        "open_loop": line(opened, before.astype(np.float64)),
+        "memory": mem,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +192,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="f8",
                     help="f8: the headline 8 KB F8 WSYNC ROM; fixed4k / branchy16k: the other ROM classes")
+    ap.add_argument("--no-mix", action="store_true", help="skip the heterogeneous-batch line")
+    ap.add_argument("--mix-launches", type=int, default=3)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving line")
+    ap.add_argument("--serve-seconds", type=float, default=6.0, help="open-loop serving time")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy kernel of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -175,6 +295,17 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved_gbs / HBM_PEAK_GBS, 6), "traffic": None},
     }
+    gpu.close()
+    d_act.free()
+    d_obs.free()
+    if not args.no_mix:
+        log("bench: heterogeneous batch (4 ROM classes, K=8, autotuned)")
+        out["mix"] = bench_mix(args, fastatari, W)
+    if not args.no_serve:
+        log("bench: serving (256 clients, continuous batching)")
+        out["served"] = bench_serve(args, fastatari, W)
+        ol = out["served"]["open_loop"]
+        out["served_p50_ms"], out["served_p99_ms"] = ol["p50_ms"], ol["p99_ms"]
     if not args.no_cpu:
         sample_n, sample_frames = 8192, 48
         cpu_fps, secs = cpu_baseline(rom, sample_n, sample_frames, seed, args.cpu_threads)

@@ -19,6 +19,15 @@ constexpr uint32_t kStateBytes = 222;     // AtariStateGpu
 constexpr uint32_t kLdsRom0 = 2048;       // LDS offset of the ROM image (after the descriptor table)
 constexpr uint32_t kMaxBlock = 256;
 
+// One workgroup's share of a served launch: a client's slot range and its
+// packed inputs / obs in pinned host memory, which the kernel reads and writes
+// directly (no copies on the serving path).
+struct HostTask {
+    uint32_t first, count;        // slots (first % 8 == 0, count <= block)
+    const uint32_t *in;           // count packed input words
+    uint8_t *obs;                 // count * 128 bytes
+};
+
 // Arguments of one frame-step launch.  Envs are addressed by slot index into
 // the AoS state array and the per-env counter arrays.
 struct StepArgs {
@@ -38,6 +47,7 @@ struct StepArgs {
                                   //   (one step; replaces actions / fs / reset); NULL: none
     const uint32_t *tasks;        // NULL: workgroup b covers slots [b * bs, b * bs + bs) of [0, n);
                                   // else (first, count) per workgroup, first % 8 == 0, count <= bs
+    const HostTask *htasks;       // one per workgroup; replaces tasks, packed and obs when set
     const uint32_t *desc;         // 512 words: opcode descriptors
     const uint8_t *rom;           // ROM image, rom_bytes (multiple of 4096)
     uint32_t rom_bytes;

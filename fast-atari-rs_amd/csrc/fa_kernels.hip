@@ -83,12 +83,24 @@ __global__ void __launch_bounds__(256) fa_frame_kernel(StepArgs g) {
     const uint32_t bs = blockDim.x;
     const uint32_t tid = threadIdx.x;
     uint32_t first, count;
-    if (g.tasks) {
+    const uint32_t *pk_in = nullptr;   // packed inputs of this workgroup's envs
+    uint8_t *obs_out = nullptr;        // obs of this workgroup's envs (last step)
+    if (g.htasks) {
+        const HostTask t = g.htasks[blockIdx.x];
+        first = t.first;
+        count = t.count;
+        pk_in = t.in;
+        obs_out = t.obs;
+    } else if (g.tasks) {
         first = g.tasks[2u * blockIdx.x];
         count = g.tasks[2u * blockIdx.x + 1u];
     } else {
         first = blockIdx.x * bs;
         count = g.n > first ? (g.n - first < bs ? g.n - first : bs) : 0u;
+    }
+    if (!g.htasks) {
+        if (g.packed) pk_in = g.packed + first;
+        if (g.obs && !g.obs_every) obs_out = g.obs + (size_t)first * 128u;
     }
     // The AoS block is staged in whole 16-byte granules: 8 envs = 1776 bytes.
     const uint32_t staged = (count + 7u) & ~7u;
@@ -116,8 +128,8 @@ __global__ void __launch_bounds__(256) fa_frame_kernel(StepArgs g) {
         const uint32_t im = img0 + tid * STATE_BYTES;
         uint64_t cpu0, fcyc0, instr0;
         uint32_t st;
-        const uint32_t pk = g.packed ? g.packed[idx] : 0u;
-        const bool restart = g.packed ? ((pk >> 16) & 0xFFu) != 0u : (g.reset && g.reset[idx]);
+        const uint32_t pk = pk_in ? pk_in[tid] : 0u;
+        const bool restart = pk_in ? ((pk >> 16) & 0xFFu) != 0u : (g.reset && g.reset[idx]);
         if (restart) {
             // restart: the ROM's reset snapshot and its counters
             const uint16_t *src = reinterpret_cast<const uint16_t *>(g.reset_state);
@@ -139,7 +151,7 @@ __global__ void __launch_bounds__(256) fa_frame_kernel(StepArgs g) {
         for (uint32_t s = 0; s < g.k; ++s) {
             const size_t at = (size_t)s * g.stride + idx;
             uint32_t nfr;
-            if (g.packed) {
+            if (pk_in) {
                 if (!(pk & (1u << 24))) e.set_action(pk & 0xFFu);
                 nfr = (pk >> 8) & 0xFFu;
             } else {
@@ -168,14 +180,14 @@ __global__ void __launch_bounds__(256) fa_frame_kernel(StepArgs g) {
         const uint4 *ssrc = reinterpret_cast<const uint4 *>(fa_lds + img0);
         for (uint32_t i = tid; i < img_bytes / 16u; i += bs) sdst[i] = ssrc[i];
     }
-    if (g.obs && !g.obs_every) {
-        // obs[slot] = RAM bytes 8..135 of the env's image; 8 lanes per env
+    if (obs_out) {
+        // obs of env j = RAM bytes 8..135 of its image; 8 lanes per env
         for (uint32_t c = tid; c < count * 8u; c += bs) {
             const uint32_t env = c >> 3, part = c & 7u;
             const uint32_t off = img0 + env * STATE_BYTES + S_RAM + part * 16u;
             uint32_t w[4];
             for (int j = 0; j < 4; ++j) w[j] = ld16(off + 4u * j) | (ld16(off + 4u * j + 2u) << 16);
-            reinterpret_cast<uint4 *>(g.obs + (size_t)(first + env) * 128u)[part] = make_uint4(w[0], w[1], w[2], w[3]);
+            reinterpret_cast<uint4 *>(obs_out + (size_t)env * 128u)[part] = make_uint4(w[0], w[1], w[2], w[3]);
         }
     }
 }

@@ -12,10 +12,16 @@
 // and queues it.  The dispatcher thread takes every queued request of the
 // chunk with the oldest waiting request, builds a task list (one workgroup
 // per <= 256 envs of a client range) and enqueues on a free launch context of
-// that class: one H2D copy per request, the task list, the frame kernel, one
-// D2H copy of obs per request, then a host callback that completes the
-// requests.  Each class has kCtxPerClass contexts (streams), so launches of
-// one class overlap on the device with each other and with other classes.
+// that class: the frame kernel alone, reading each client's packed inputs
+// from and writing its obs to the client's pinned host buffers (zero-copy:
+// no copy-engine work, which would serialise across classes), then an event.  Each class has one stream (the device exposes few hardware
+// queues per process, and kernels of streams that share a queue serialise, so
+// one stream per class keeps a slow class from delaying a fast one) and
+// kCtxPerClass launch contexts on it; classes overlap on the device.  A launch
+// ends with an event; one completer thread per class waits for its launches
+// in order and completes their requests, so a slow class never delays the
+// completions of a fast one.  With one context per class, the next launch of
+// a class takes every request that arrived while the previous one ran.
 // A client has at most one request in flight, so concurrent launches touch
 // disjoint slots.
 #include <hip/hip_runtime.h>
@@ -27,8 +33,10 @@
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <random>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "fa_internal.hpp"
@@ -41,7 +49,7 @@ using Clock = std::chrono::steady_clock;
 constexpr uint32_t kChunkSlots = 65536;
 constexpr uint32_t kUnit = 64;                 // slot allocation granule
 constexpr uint32_t kUnits = kChunkSlots / kUnit;
-constexpr uint32_t kCtxPerClass = 3;
+constexpr uint32_t kCtxPerClass = 1;
 constexpr uint32_t kTaskEnvs = 256;            // envs per workgroup
 constexpr uint32_t kMaxClientEnvs = kChunkSlots;
 constexpr size_t kSlotBytes = fa::kStateBytes + 3 * 8 + 4 + 4 + 128;
@@ -60,6 +68,8 @@ struct Client {
     uint32_t first = 0, n = 0, units = 0;
     uint32_t *h_in = nullptr;    // pinned: packed inputs, n words
     uint8_t *h_obs = nullptr;    // pinned: n * 128
+    uint32_t *dv_in = nullptr;   // device addresses of the two
+    uint8_t *dv_obs = nullptr;
     bool open = false;
     int state = 0;               // 0 idle, 1 queued, 2 in flight, 3 done
     uint32_t frames = 0;
@@ -72,8 +82,10 @@ struct ServerClass;
 struct LaunchCtx {
     fa_server *srv = nullptr;
     int cls = 0;
-    hipStream_t stream = nullptr;
-    uint32_t *d_tasks = nullptr, *h_tasks = nullptr;
+    hipStream_t stream = nullptr;      // the class's stream (not owned)
+    hipEvent_t done = nullptr;         // recorded after the launch's obs copies
+    fa::HostTask *h_tasks = nullptr;   // pinned, read by the kernel
+    fa::HostTask *dv_tasks = nullptr;  // its device address
     size_t task_cap = 0;
     std::vector<int> clients;
     bool busy = false;
@@ -86,6 +98,10 @@ struct ServerClass {
     std::vector<std::unique_ptr<Chunk>> chunks;
     std::vector<std::unique_ptr<LaunchCtx>> ctx;
     std::deque<int> queue;             // clients with a queued request
+    std::deque<LaunchCtx *> inflight;  // launches on the stream, oldest first
+    hipStream_t stream = nullptr;
+    uint32_t open_clients = 0;
+    Clock::time_point hold_until{};    // coalescing window after a completion
     uint32_t block = 256;
     int sorted = 0;
 };
@@ -102,6 +118,7 @@ struct fa_server {
     int device = 0;
     size_t max_lds = 65536;
     size_t budget = 0, used_bytes = 0;
+    uint32_t coalesce_us = 300;
     uint32_t *d_desc = nullptr;
     std::deque<ServerClass> cls;       // deque: ServerClass is move-only
     std::deque<Client> clients;        // stable addresses: the dispatcher reads them unlocked
@@ -113,11 +130,15 @@ struct fa_server {
     std::mutex mu;
     std::condition_variable cv_dispatch, cv_done;
     std::thread dispatcher;
-    bool stop = false;
+    std::vector<std::thread> completers;   // one per class
+    std::condition_variable cv_inflight;
+    bool stop = false, stop_completers = false;
     int error = FA_OK;
     std::string error_msg;
     std::vector<Sample> samples;
     uint64_t launches = 0, env_frames = 0, requests = 0;
+    std::vector<int> completions;      // finished requests, drained by the load driver
+    bool track_completions = false;
 };
 
 namespace {
@@ -189,22 +210,25 @@ uint32_t fill_tasks(uint32_t *tasks, uint32_t first, uint32_t n) {
     return t;
 }
 
-void on_launch_done(void *arg);
+void finish_launch(LaunchCtx *ctx);
 
 // Dispatcher: one launch per iteration; waits for queued work and a free
-// launch context of the same class.
+// launch context of the same class.  After a class's launch completes, its
+// clients resubmit one by one: the class is held for up to coalesce_us (or
+// until every open client of the class has a request queued) so that the
+// next launch takes them together instead of running a near-empty launch
+// first.
 void dispatch_loop(fa_server *s) {
     std::unique_lock<std::mutex> lk(s->mu);
     size_t rr = 0;
     while (true) {
         int pick = -1;
         LaunchCtx *ctx = nullptr;
-        s->cv_dispatch.wait(lk, [&] {
-            if (s->stop) return true;
-            // the class whose head request waited longest and has a free context
-            Clock::time_point best{};
-            pick = -1;
-            ctx = nullptr;
+        while (!s->stop) {
+            // the eligible class whose head request waited longest
+            Clock::time_point best{}, wake{};
+            bool held = false;
+            const auto now = Clock::now();
             for (size_t i = 0; i < s->cls.size(); ++i) {
                 const size_t ci = (rr + i) % s->cls.size();
                 auto &sc = s->cls[ci];
@@ -216,6 +240,11 @@ void dispatch_loop(fa_server *s) {
                         break;
                     }
                 if (!free_ctx) continue;
+                if (now < sc.hold_until && sc.queue.size() < sc.open_clients) {
+                    if (!held || sc.hold_until < wake) wake = sc.hold_until;
+                    held = true;
+                    continue;
+                }
                 const auto t = s->clients[sc.queue.front()].t_submit;
                 if (pick < 0 || t < best) {
                     best = t;
@@ -223,8 +252,12 @@ void dispatch_loop(fa_server *s) {
                     ctx = free_ctx;
                 }
             }
-            return pick >= 0;
-        });
+            if (pick >= 0) break;
+            if (held)
+                s->cv_dispatch.wait_until(lk, wake);
+            else
+                s->cv_dispatch.wait(lk);
+        }
         if (s->stop) return;
         rr = (size_t)pick + 1;
         ServerClass &sc = s->cls[pick];
@@ -257,40 +290,31 @@ void dispatch_loop(fa_server *s) {
             }
             return e == hipSuccess;
         };
-        if (ntasks * 2 > ctx->task_cap) {
-            // grows rarely: the first launches of a context
-            hip_ok(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-            if (ctx->d_tasks) hipFree(ctx->d_tasks);
+        if (ntasks > ctx->task_cap) {
+            // grows rarely: the first launches of a context (idle: its last launch completed)
             if (ctx->h_tasks) hipHostFree(ctx->h_tasks);
-            ctx->d_tasks = nullptr;
-            ctx->h_tasks = nullptr;
-            const size_t cap = std::max<size_t>(ntasks * 2 * 2, 1024);
-            if (hip_ok(hipMalloc(&ctx->d_tasks, cap * 4), "hipMalloc") &&
-                hip_ok(hipHostMalloc(&ctx->h_tasks, cap * 4, hipHostMallocDefault), "hipHostMalloc"))
+            ctx->h_tasks = ctx->dv_tasks = nullptr;
+            ctx->task_cap = 0;
+            const size_t cap = std::max<size_t>(ntasks * 2, 512);
+            if (hip_ok(hipHostMalloc(&ctx->h_tasks, cap * sizeof(fa::HostTask), hipHostMallocDefault), "hipHostMalloc") &&
+                hip_ok(hipHostGetDevicePointer((void **)&ctx->dv_tasks, ctx->h_tasks, 0), "hipHostGetDevicePointer"))
                 ctx->task_cap = cap;
-            else
-                ctx->task_cap = 0;
         }
         if (rc == FA_OK) {
+            // one workgroup per <= 256 envs of each client; inputs and obs stay in the client's pinned buffers
             uint32_t t = 0;
             for (int id : ctx->clients) {
-                Client &cl = s->clients[id];
-                t += fill_tasks(ctx->h_tasks + 2 * t, cl.first, cl.n);
-                hip_ok(hipMemcpyAsync(ch.d_in + cl.first, cl.h_in, (size_t)cl.n * 4, hipMemcpyHostToDevice, ctx->stream),
-                       "input copy");
+                const Client &cl = s->clients[id];
+                for (uint32_t off = 0; off < cl.n; off += kTaskEnvs)
+                    ctx->h_tasks[t++] = fa::HostTask{cl.first + off, std::min(kTaskEnvs, cl.n - off), cl.dv_in + off,
+                                                     cl.dv_obs + (size_t)off * 128u};
             }
-            hip_ok(hipMemcpyAsync(ctx->d_tasks, ctx->h_tasks, (size_t)t * 8, hipMemcpyHostToDevice, ctx->stream),
-                   "task copy");
             fa::StepArgs g = chunk_args(s, sc, ch);
-            g.tasks = ctx->d_tasks;
-            if (rc == FA_OK) hip_ok(fa::launch_step(g, sc.sorted != 0, sc.block, t, ctx->stream), "frame kernel");
-            for (int id : ctx->clients) {
-                Client &cl = s->clients[id];
-                hip_ok(hipMemcpyAsync(cl.h_obs, ch.d_obs + (size_t)cl.first * 128, (size_t)cl.n * 128,
-                                      hipMemcpyDeviceToHost, ctx->stream),
-                       "obs copy");
-            }
-            hip_ok(hipLaunchHostFunc(ctx->stream, on_launch_done, ctx), "hipLaunchHostFunc");
+            g.packed = nullptr;
+            g.obs = nullptr;
+            g.htasks = ctx->dv_tasks;
+            hip_ok(fa::launch_step(g, sc.sorted != 0, sc.block, t, ctx->stream), "frame kernel");
+            hip_ok(hipEventRecord(ctx->done, ctx->stream), "hipEventRecord");
         }
         lk.lock();
         if (rc != FA_OK) {
@@ -302,13 +326,35 @@ void dispatch_loop(fa_server *s) {
             s->cv_done.notify_all();
         } else {
             s->launches++;
+            sc.inflight.push_back(ctx);
+            s->cv_inflight.notify_all();
         }
     }
 }
 
-// Runs on a HIP runtime thread once a launch's obs are in host memory.
-void on_launch_done(void *arg) {
-    auto *ctx = static_cast<LaunchCtx *>(arg);
+// Completer of one class: waits for the class's launches in stream order.
+void complete_loop(fa_server *s, size_t ci) {
+    ServerClass &sc = s->cls[ci];
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (true) {
+        s->cv_inflight.wait(lk, [&] { return s->stop_completers || !sc.inflight.empty(); });
+        if (sc.inflight.empty()) return;          // stopping, nothing left
+        LaunchCtx *ctx = sc.inflight.front();
+        lk.unlock();
+        const hipError_t e = hipEventSynchronize(ctx->done);
+        if (e != hipSuccess) {
+            std::lock_guard<std::mutex> g(s->mu);
+            s->error = FA_EHIP;
+            s->error_msg = std::string("launch failed: ") + hipGetErrorString(e);
+        }
+        finish_launch(ctx);
+        lk.lock();
+        sc.inflight.pop_front();
+    }
+}
+
+// A launch's obs are in host memory: complete its requests.
+void finish_launch(LaunchCtx *ctx) {
     fa_server *s = ctx->srv;
     const auto now = Clock::now();
     {
@@ -320,8 +366,10 @@ void on_launch_done(void *arg) {
             s->samples.push_back({cl.latency_ms, cl.cls, cl.n, cl.frames});
             s->env_frames += (uint64_t)cl.n * cl.frames;
             s->requests++;
+            if (s->track_completions) s->completions.push_back(id);
         }
         ctx->busy = false;
+        s->cls[ctx->cls].hold_until = now + std::chrono::microseconds(s->coalesce_us);
     }
     s->cv_done.notify_all();
     s->cv_dispatch.notify_all();
@@ -440,17 +488,23 @@ int fa_server_create(const uint8_t *const *roms, const size_t *rom_lens, size_t 
             return cleanup(fail(FA_EHIP, "ROM upload failed"));
         int rc = make_snapshot(s, sc);
         if (rc) return cleanup(rc);
+        if (hipStreamCreateWithFlags(&sc.stream, hipStreamNonBlocking) != hipSuccess)
+            return cleanup(fail(FA_EHIP, "stream creation failed"));
         for (uint32_t i = 0; i < kCtxPerClass; ++i) {
             auto ctx = std::make_unique<LaunchCtx>();
             ctx->srv = s;
             ctx->cls = (int)c;
-            if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
-                return cleanup(fail(FA_EHIP, "stream creation failed"));
+            ctx->stream = sc.stream;
+            if (hipEventCreateWithFlags(&ctx->done, hipEventDisableTiming | hipEventBlockingSync) != hipSuccess) {
+                sc.ctx.push_back(std::move(ctx));
+                return cleanup(fail(FA_EHIP, "event creation failed"));
+            }
             sc.ctx.push_back(std::move(ctx));
         }
         sc.block = kTaskEnvs;
     }
     s->dispatcher = std::thread(dispatch_loop, s);
+    for (size_t c = 0; c < nclass; ++c) s->completers.emplace_back(complete_loop, s, c);
     *out = s;
     return FA_OK;
 }
@@ -463,15 +517,21 @@ void fa_server_destroy(fa_server *s) {
     }
     s->cv_dispatch.notify_all();
     if (s->dispatcher.joinable()) s->dispatcher.join();
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->stop_completers = true;
+    }
+    s->cv_inflight.notify_all();
+    for (auto &t : s->completers)
+        if (t.joinable()) t.join();
     for (auto &sc : s->cls)
-        for (auto &c : sc.ctx)
-            if (c->stream) hipStreamSynchronize(c->stream);
+        if (sc.stream) hipStreamSynchronize(sc.stream);
     for (auto &sc : s->cls) {
         for (auto &c : sc.ctx) {
-            if (c->d_tasks) hipFree(c->d_tasks);
             if (c->h_tasks) hipHostFree(c->h_tasks);
-            if (c->stream) hipStreamDestroy(c->stream);
+            if (c->done) hipEventDestroy(c->done);
         }
+        if (sc.stream) hipStreamDestroy(sc.stream);
         for (auto &ch : sc.chunks) free_chunk_bytes(*ch);
         if (sc.d_rom) hipFree(sc.d_rom);
         if (sc.d_snap) hipFree(sc.d_snap);
@@ -533,11 +593,13 @@ int fa_server_open(fa_server *s, size_t cls, size_t n_envs, uint8_t *obs_out, in
     cl.n = (uint32_t)n_envs;
     cl.units = units;
     cl.open = true;
+    sc.open_clients++;
     lk.unlock();
     auto undo = [&](int code) {
         std::lock_guard<std::mutex> g(s->mu);
         for (uint32_t u = 0; u < units; ++u) ch.used[at + u] = 0;
         ch.used_units -= units;
+        sc.open_clients--;
         if (cl.h_in) hipHostFree(cl.h_in);
         if (cl.h_obs) hipHostFree(cl.h_obs);
         cl = Client{};
@@ -547,6 +609,9 @@ int fa_server_open(fa_server *s, size_t cls, size_t n_envs, uint8_t *obs_out, in
     if (hipHostMalloc(&cl.h_in, (size_t)n_envs * 4, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc(&cl.h_obs, (size_t)n_envs * 128, hipHostMallocDefault) != hipSuccess)
         return undo(fail(FA_ENOMEM, "pinned staging allocation failed"));
+    if (hipHostGetDevicePointer((void **)&cl.dv_in, cl.h_in, 0) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&cl.dv_obs, cl.h_obs, 0) != hipSuccess)
+        return undo(fail(FA_EHIP, "pinned staging is not device-accessible"));
     int rc = init_range(s, sc, ch, cl);
     if (rc) return undo(rc);
     if (obs_out) std::memcpy(obs_out, cl.h_obs, (size_t)n_envs * 128);
@@ -561,6 +626,7 @@ int fa_server_close(fa_server *s, int id) {
     Client &cl = s->clients[id];
     s->cv_done.wait(lk, [&] { return cl.state == 0 || cl.state == 3; });
     Chunk &ch = *s->cls[cl.cls].chunks[cl.chunk];
+    s->cls[cl.cls].open_clients--;
     const uint32_t u0 = cl.first / kUnit;
     for (uint32_t u = 0; u < cl.units; ++u) ch.used[u0 + u] = 0;
     ch.used_units -= cl.units;
@@ -649,6 +715,14 @@ int fa_server_client_info(fa_server *s, int id, size_t *cls, size_t *n_envs) {
     return FA_OK;
 }
 
+int fa_server_set_coalesce_us(fa_server *s, uint32_t us) {
+    if (!s) return fail(FA_EINVAL, "NULL server");
+    if (us > 1000000) return fail(FA_EINVAL, "coalescing window above 1 s");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->coalesce_us = us;
+    return FA_OK;
+}
+
 int fa_server_memory(fa_server *s, size_t *used_bytes, size_t *budget_bytes, size_t *bytes_per_slot) {
     if (!s) return fail(FA_EINVAL, "NULL server");
     std::lock_guard<std::mutex> lk(s->mu);
@@ -678,6 +752,163 @@ int fa_server_samples(fa_server *s, double *ms, uint32_t *cls, uint32_t *envs, u
         s->launches = 0;
     }
     return FA_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Load driver: stands in for the clients.  Per client, actions and restart
+// masks come from per-env SplitMix64 streams (the host package's
+// ActionStream: state_i = mix(seed ^ i * GOLDEN + GOLDEN), step s uses
+// mix(state_i + (s + 1) * GOLDEN); action = (u64 * 18) >> 64; restart when the
+// restart stream's u64 < restart_p * 2^64), so any step of any env can be
+// regenerated from the client's seed and step count.  Closed loop (sat ==
+// NULL): every client resubmits as soon as its request completes.  Open loop:
+// Poisson arrivals at rate_mult[i] * sat[i] per second; arrivals wait in the
+// client's backlog while its previous request is in flight.
+static inline uint64_t sm_mix(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
+static constexpr uint64_t kRestartSalt = 0x5E5E7000ull;
+
+int fa_loadgen_run(fa_server *s, const int *client_ids, const fa_client_spec *specs, size_t nclients,
+                   double duration_s, const double *sat, uint32_t frames, double restart_p, uint64_t arrival_seed,
+                   uint64_t max_steps, uint64_t *steps, double *service_ms, double *e2e_ms, uint32_t *sample_cls,
+                   size_t cap, size_t *nsamples, double *wall_s) {
+    if (!s || !client_ids || !specs || !steps) return fail(FA_EINVAL, "NULL argument");
+    if (frames > 255 || restart_p < 0.0 || restart_p > 1.0) return fail(FA_EINVAL, "bad frames or restart_p");
+    std::unordered_map<int, size_t> index;
+    for (size_t i = 0; i < nclients; ++i) {
+        int rc = check_client(s, client_ids[i]);
+        if (rc) return rc;
+        index[client_ids[i]] = i;
+    }
+    const uint64_t thresh =
+        restart_p >= 1.0 ? ~0ull : (uint64_t)(restart_p * 18446744073709551616.0);
+    std::mt19937_64 rng(arrival_seed);
+    std::vector<std::exponential_distribution<double>> gap;
+    for (size_t i = 0; i < nclients; ++i) {
+        double lam = sat ? std::max(1e-9, specs[i].rate_mult * sat[i]) : 1.0;
+        gap.emplace_back(lam);
+    }
+    using dsec = std::chrono::duration<double>;
+    const auto t0 = Clock::now();
+    const auto t_end = t0 + std::chrono::duration_cast<Clock::duration>(dsec(duration_s));
+    std::vector<Clock::time_point> next_arr(nclients, t0);
+    if (sat)
+        for (size_t i = 0; i < nclients; ++i)
+            next_arr[i] = t0 + std::chrono::duration_cast<Clock::duration>(dsec(gap[i](rng)));
+    std::vector<std::deque<Clock::time_point>> backlog(nclients);
+    std::vector<Clock::time_point> arrival(nclients);
+    std::vector<char> inflight(nclients, 0);
+    size_t n_inflight = 0, ns = 0;
+    std::vector<uint8_t> act, rst;
+    std::vector<int> done;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->completions.clear();
+        s->track_completions = true;
+    }
+    int result = FA_OK;
+    while (true) {
+        auto now = Clock::now();
+        const bool open = now < t_end;
+        if (open) {
+            for (size_t i = 0; i < nclients; ++i) {
+                if (sat) {
+                    while (next_arr[i] <= now) {
+                        backlog[i].push_back(next_arr[i]);
+                        next_arr[i] += std::chrono::duration_cast<Clock::duration>(dsec(gap[i](rng)));
+                    }
+                } else if (!inflight[i] && backlog[i].empty()) {
+                    backlog[i].push_back(now);
+                }
+            }
+        } else if (n_inflight == 0) {
+            break;
+        }
+        // submit for every idle client with a waiting arrival
+        for (size_t i = 0; open && i < nclients; ++i) {
+            if (inflight[i] || backlog[i].empty()) continue;
+            if (max_steps && steps[i] >= max_steps) {
+                backlog[i].clear();
+                continue;
+            }
+            const fa_client_spec &sp = specs[i];
+            act.resize(sp.n);
+            rst.resize(sp.n);
+            const uint64_t st = steps[i] + 1;
+            for (uint32_t e = 0; e < sp.n; ++e) {
+                const uint64_t a0 = sm_mix((sp.seed ^ ((uint64_t)e * kGolden)) + kGolden);
+                const uint64_t r0 = sm_mix(((sp.seed ^ kRestartSalt) ^ ((uint64_t)e * kGolden)) + kGolden);
+                const uint64_t za = sm_mix(a0 + st * kGolden);
+                const uint64_t zr = sm_mix(r0 + st * kGolden);
+                act[e] = (uint8_t)(((unsigned __int128)za * 18u) >> 64);
+                rst[e] = zr < thresh ? 1 : 0;
+            }
+            int rc = fa_server_submit(s, client_ids[i], act.data(), rst.data(), frames);
+            if (rc) {
+                result = rc;
+                break;
+            }
+            arrival[i] = backlog[i].front();
+            backlog[i].pop_front();
+            inflight[i] = 1;
+            ++n_inflight;
+        }
+        if (result) break;
+        // completions
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            if (s->completions.empty()) {
+                auto until = t_end;
+                if (now < t_end && sat)
+                    for (size_t i = 0; i < nclients; ++i) until = std::min(until, next_arr[i]);
+                if (now >= t_end) until = now + std::chrono::milliseconds(5);
+                s->cv_done.wait_until(lk, until, [&] { return !s->completions.empty() || s->error; });
+            }
+            done.swap(s->completions);
+            s->completions.clear();
+            if (s->error) {
+                result = s->error;
+                fail(s->error, s->error_msg);
+            }
+            const auto t_done = Clock::now();
+            for (int id : done) {
+                auto it = index.find(id);
+                if (it == index.end()) continue;
+                const size_t i = it->second;
+                Client &cl = s->clients[id];
+                cl.state = 0;
+                if (ns < cap) {
+                    if (service_ms) service_ms[ns] = cl.latency_ms;
+                    if (e2e_ms) e2e_ms[ns] = std::chrono::duration<double, std::milli>(t_done - arrival[i]).count();
+                    if (sample_cls) sample_cls[ns] = (uint32_t)cl.cls;
+                }
+                ++ns;
+                inflight[i] = 0;
+                --n_inflight;
+                ++steps[i];
+            }
+            done.clear();
+        }
+        if (result) break;
+        if (max_steps && n_inflight == 0) {
+            bool all = true;
+            for (size_t i = 0; i < nclients && all; ++i) all = steps[i] >= max_steps;
+            if (all) break;
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->track_completions = false;
+        s->completions.clear();
+    }
+    if (nsamples) *nsamples = ns;
+    if (wall_s) *wall_s = std::chrono::duration<double>(Clock::now() - t0).count();
+    return result;
 }
 
 }  // extern "C"

@@ -142,8 +142,13 @@ def load_library():
         "fa_server_client_info": (C.c_int, [vp, C.c_int, C.POINTER(sz), C.POINTER(sz)]),
         "fa_server_set_class_config": (C.c_int, [vp, sz, C.c_uint32, C.c_int]),
         "fa_server_memory": (C.c_int, [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]),
+        "fa_server_set_coalesce_us": (C.c_int, [vp, C.c_uint32]),
         "fa_server_samples": (C.c_int, [vp, C.POINTER(C.c_double), u32p, u32p, u32p, sz, C.POINTER(sz), u64p,
                                         C.c_int]),
+        "fa_loadgen_run": (C.c_int, [vp, C.POINTER(C.c_int), vp, sz, C.c_double, C.POINTER(C.c_double),
+                                     C.c_uint32, C.c_double, C.c_uint64, C.c_uint64, u64p,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double), u32p, sz, C.POINTER(sz),
+                                     C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)

@@ -1,9 +1,11 @@
 """Serving workload: 256 clients on the four mixed ROMs, each owning
 floor(64 * 32**u) envs (u ~ U[0, 1)), stepping with Poisson arrivals at
 r x its saturation rate (r ~ U[0.5, 2]) and restarting ~1 % of envs per
-step.  Drives an EnvServer from one thread and records per-request service
-latency (submit -> obs in host memory, measured natively) and end-to-end
-latency (arrival -> obs, including the client's own backlog).
+step.  run_load drives an EnvServer from Python (tests at small scale);
+EnvServer.run_load is the native driver with the same streams, used at full
+scale.  Both record per-request service latency (submit -> obs in host
+memory, measured natively) and end-to-end latency (arrival -> obs, including
+the client's own backlog).
 
 Every client's results depend only on its own request sequence (actions,
 restarts, frames), which is drawn from the client's seed, so a client run
@@ -16,7 +18,15 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .workloads import GOLDEN, MASK64, ActionStream, Rng, splitmix64
+from .workloads import GOLDEN, MASK64, ActionStream, Rng, _mulhi, splitmix64
+
+
+def _mix(z: np.ndarray) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
 
 RESTART_P = 0.01
 
@@ -53,6 +63,19 @@ class ClientStream:
         self.threshold = np.uint64(int(RESTART_P * 2 ** 64))
         self.record = record        # env indices whose inputs are kept
         self.history = []
+
+    def inputs_at(self, steps: int, envs):
+        """Actions and restart masks of steps 0..steps-1 for the given envs
+        ([steps, len(envs)] each), by jumping the SplitMix64 streams ahead:
+        step s of env i draws mix(state_i + (s + 1) * GOLDEN)."""
+        envs = np.asarray(envs, dtype=np.int64)
+        fresh_a = ActionStream(self.spec.seed, self.spec.n)
+        fresh_r = ActionStream(self.spec.seed ^ 0x5E5E7000, self.spec.n)
+        k = np.arange(1, steps + 1, dtype=np.uint64)[:, None]
+        with np.errstate(over="ignore"):
+            za = _mix(fresh_a.state[envs][None, :] + k * np.uint64(GOLDEN))
+            zr = _mix(fresh_r.state[envs][None, :] + k * np.uint64(GOLDEN))
+        return _mulhi(za, 18).astype(np.uint8), (zr < self.threshold).astype(np.uint8)
 
     def next(self):
         a = self.actions.next()

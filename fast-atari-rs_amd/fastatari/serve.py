@@ -21,6 +21,10 @@ import numpy as np
 
 from .env import STATE_DTYPE, _check, _p8, _u8, load_library
 
+# fa_client_spec (include/fastatari.h)
+SPEC_DTYPE = np.dtype({"names": ["cls", "n", "rate_mult", "seed"], "formats": ["<u4", "<u4", "<f8", "<u8"],
+                       "offsets": [0, 4, 8, 16], "itemsize": 24})
+
 
 class EnvServer:
     def __init__(self, roms, budget_bytes: int = 64 << 30):
@@ -98,10 +102,42 @@ class EnvServer:
     def set_class_config(self, cls: int, sorted_: bool = False):
         _check(load_library().fa_server_set_class_config(self._h, int(cls), 256, int(bool(sorted_))))
 
+    def set_coalesce_us(self, us: int):
+        _check(load_library().fa_server_set_coalesce_us(self._h, int(us)))
+
     def memory(self) -> dict:
         u, b, s = C.c_size_t(), C.c_size_t(), C.c_size_t()
         _check(load_library().fa_server_memory(self._h, C.byref(u), C.byref(b), C.byref(s)))
         return {"used_bytes": u.value, "budget_bytes": b.value, "bytes_per_slot": s.value}
+
+    def run_load(self, cids, specs, duration_s: float, sat=None, frames: int = 1, restart_p: float = 0.01,
+                 arrival_seed: int = 1, max_steps: int = 0, steps=None, cap: int = 1 << 20) -> dict:
+        """Native load driver (fa_loadgen_run): specs are loadgen.ClientSpec;
+        steps (uint64 per client, updated in place) counts each client's
+        completed steps, which fixes its next actions / restarts."""
+        nc = len(cids)
+        spec_arr = np.zeros(nc, SPEC_DTYPE)
+        for i, sp in enumerate(specs):
+            spec_arr[i] = (sp.cls, sp.n, sp.rate_mult, sp.seed & ((1 << 64) - 1))
+        ids = (C.c_int * nc)(*[int(c) for c in cids])
+        if steps is None:
+            steps = np.zeros(nc, np.uint64)
+        sat_arr = None if sat is None else np.ascontiguousarray(sat, dtype=np.float64)
+        svc = np.zeros(cap, np.float64)
+        e2e = np.zeros(cap, np.float64)
+        cls = np.zeros(cap, np.uint32)
+        ns = C.c_size_t()
+        wall = C.c_double()
+        pd = C.POINTER(C.c_double)
+        _check(load_library().fa_loadgen_run(
+            self._h, ids, spec_arr.ctypes.data_as(C.c_void_p), nc, float(duration_s),
+            sat_arr.ctypes.data_as(pd) if sat_arr is not None else None, int(frames), float(restart_p),
+            int(arrival_seed), int(max_steps), steps.ctypes.data_as(C.POINTER(C.c_uint64)),
+            svc.ctypes.data_as(pd), e2e.ctypes.data_as(pd), cls.ctypes.data_as(C.POINTER(C.c_uint32)), cap,
+            C.byref(ns), C.byref(wall)))
+        m = min(ns.value, cap)
+        return {"service_ms": svc[:m], "e2e_ms": e2e[:m], "cls": cls[:m], "requests": ns.value,
+                "wall_s": wall.value, "steps": steps}
 
     def samples(self, reset: bool = False) -> dict:
         cnt = C.c_size_t()

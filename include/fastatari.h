@@ -221,11 +221,32 @@ int fa_server_download(fa_server *s, int client_id, AtariStateGpu *states, uint6
                        uint64_t *frame_cycles, uint32_t *status);
 int fa_server_client_info(fa_server *s, int client_id, size_t *cls, size_t *n_envs);
 int fa_server_set_class_config(fa_server *s, size_t cls, uint32_t block, int sorted);
+/* Coalescing window (default 300 us): after a launch of a class completes,
+ * its next launch waits up to this long for the class's clients to resubmit
+ * (it starts at once when all of them have). */
+int fa_server_set_coalesce_us(fa_server *s, uint32_t us);
 int fa_server_memory(fa_server *s, size_t *used_bytes, size_t *budget_bytes, size_t *bytes_per_slot);
 /* Per completed request since the last reset: latency ms, class, envs,
  * frames; *count = samples held; *launches = kernel launches. */
 int fa_server_samples(fa_server *s, double *ms, uint32_t *cls, uint32_t *envs, uint32_t *frames, size_t cap,
                       size_t *count, uint64_t *launches, int reset);
+
+/* Load driver standing in for the clients (benchmarks, tests).  specs[i]
+ * describes client_ids[i]: its envs' actions and restarts come from per-env
+ * SplitMix64 streams of specs[i].seed, step steps[i] onwards (steps is
+ * updated).  sat == NULL: closed loop; else Poisson arrivals at
+ * rate_mult * sat[i] per second.  Per completed request: service latency
+ * (submit -> obs in host memory), end-to-end latency (arrival -> obs) and
+ * class, the first `cap` of them; *nsamples = completed requests. */
+typedef struct fa_client_spec {
+    uint32_t cls, n;
+    double rate_mult;
+    uint64_t seed;
+} fa_client_spec;
+int fa_loadgen_run(fa_server *s, const int *client_ids, const fa_client_spec *specs, size_t nclients,
+                   double duration_s, const double *sat, uint32_t frames, double restart_p, uint64_t arrival_seed,
+                   uint64_t max_steps, uint64_t *steps, double *service_ms, double *e2e_ms, uint32_t *sample_cls,
+                   size_t cap, size_t *nsamples, double *wall_s);
 
 #ifdef __cplusplus
 }

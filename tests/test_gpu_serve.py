@@ -176,3 +176,99 @@ def test_second_submit_while_in_flight_is_busy(fa_mod, roms):
     with pytest.raises(ValueError):
         srv.submit(cid, np.zeros(10, np.uint8))
     srv.close_server()
+
+
+def _replay_counts(orc, roms, snaps, specs, steps, records, frames=1):
+    """Oracle replay of the recorded envs of every client from its completed
+    step count (inputs regenerated by jumping the client's streams ahead);
+    one oracle Batch per ROM class, clients with fewer steps padded with
+    zero-frame steps repeating their last action (no state change)."""
+    from fastatari import loadgen as LG
+    out = {}
+    for c in range(len(roms)):
+        members = [i for i, sp in enumerate(specs) if sp.cls == c and steps[i] > 0 and len(records[i])]
+        for i, sp in enumerate(specs):
+            if sp.cls == c and steps[i] == 0:
+                out[i] = None
+        if not members:
+            continue
+        ins = {i: LG.ClientStream(specs[i]).inputs_at(int(steps[i]), records[i]) for i in members}
+        rows = [len(records[i]) for i in members]
+        n = sum(rows)
+        ref = orc.Batch(roms[c], n, nthreads=16)
+        ref.reset()
+        for s in range(int(max(steps[i] for i in members))):
+            a = np.zeros(n, np.uint8)
+            r = np.zeros(n, np.uint8)
+            fs = np.zeros(n, np.uint8)
+            off = 0
+            for i, m in zip(members, rows):
+                acts, rst = ins[i]
+                if s < len(acts):
+                    a[off:off + m], r[off:off + m], fs[off:off + m] = acts[s], rst[s], frames
+                else:
+                    a[off:off + m] = acts[-1]
+                off += m
+            ref.restart(r, snaps[c][0], snaps[c][1])
+            ref.run_frameskip(a, fs)
+        st, cn = ref.states(), ref.counters()
+        off = 0
+        for i, m in zip(members, rows):
+            out[i] = (st[off:off + m], cn[off:off + m])
+            off += m
+    return out
+
+
+def test_config5_full_scale_sampled(fa_mod, orc, LG, roms, snaps):
+    """The serving configuration at full scale: 256 clients (~146K envs,
+    64-2047 each) on the four ROMs, native load driver, closed loop then
+    Poisson arrivals at 0.5-2x saturation, 1 % restarts per step; 1 % of the
+    envs (every 100th of each client) replayed on the oracle bit-exactly."""
+    specs = LG.serve_clients(SEED)
+    srv = fa_mod.EnvServer(roms, budget_bytes=64 << 30)
+    cids = [srv.open(sp.cls, sp.n)[0] for sp in specs]
+    steps = np.zeros(len(specs), np.uint64)
+    closed = srv.run_load(cids, specs, 0.6, steps=steps)
+    assert closed["requests"] > 0
+    sat = np.maximum(steps, 1) / closed["wall_s"]
+    opened = srv.run_load(cids, specs, 1.0, sat=sat, steps=steps, arrival_seed=3)
+    assert opened["requests"] > 0 and (opened["service_ms"] > 0).all()
+    records = [np.arange(0, sp.n, 100) for sp in specs]
+    ref = _replay_counts(orc, roms, snaps, specs, steps, records)
+    checked = 0
+    for i, (cid, sp) in enumerate(zip(cids, specs)):
+        d = srv.download(cid)
+        if ref[i] is None:
+            assert (d["states"] == snaps[sp.cls][0]).all()
+            continue
+        st, cn = ref[i]
+        idx = records[i]
+        assert np.array_equal(d["states"][idx], st), f"client {i}: {describe_state_diff(d['states'][idx], st, idx)}"
+        assert np.array_equal(d["cpu_cycles"][idx], cn[:, 0]), f"client {i} cpu_cycles"
+        assert np.array_equal(d["frame_cycles"][idx], cn[:, 1]), f"client {i} frame_cycles"
+        assert not d["status"].any()
+        checked += len(idx)
+    assert checked >= 1000
+    srv.close_server()
+
+
+def test_native_driver_alone_vs_mix(fa_mod, LG, roms):
+    """The native driver: a client's states after S steps are the same when it
+    runs alone and when it runs inside a closed-loop mix of 40 clients."""
+    specs = LG.serve_clients(SEED + 7, n_clients=40, lo=64, ratio=16.0)
+    S = 5
+
+    def run(which):
+        srv = fa_mod.EnvServer(roms, budget_bytes=1 << 30)
+        cids = [srv.open(specs[i].cls, specs[i].n)[0] for i in which]
+        steps = np.zeros(len(which), np.uint64)
+        srv.run_load(cids, [specs[i] for i in which], 120.0, steps=steps, max_steps=S)
+        assert (steps == S).all()
+        out = [srv.download(c) for c in cids]
+        srv.close_server()
+        return out
+
+    alone = run([3])
+    mix = run(list(range(len(specs))))
+    for k in ("states", "cpu_cycles", "frame_cycles", "status"):
+        assert np.array_equal(alone[0][k], mix[3][k]), f"{k} differs alone vs in the mix"

@@ -37,3 +37,14 @@ def test_summarize():
     s = LG.summarize(res)
     assert s["fps"] == 2000.0 and s["requests"] == 4
     assert s["p50_ms"] == 2.5 and s["per_class"][0]["requests"] == 2
+
+
+def test_jump_ahead_matches_the_sequential_stream():
+    spec = LG.ClientSpec(2, 300, 1.0, 0xABCDEF)
+    seq = LG.ClientStream(spec)
+    steps = [seq.next() for _ in range(7)]
+    envs = np.array([0, 5, 299])
+    a, r = LG.ClientStream(spec).inputs_at(7, envs)
+    for s in range(7):
+        assert np.array_equal(a[s], steps[s][0][envs])
+        assert np.array_equal(r[s], steps[s][1][envs])
