@@ -31,9 +31,15 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # Algorithmic HBM bytes per env-frame of the per-frame step: read state 222 +
-# action 1, write state 222 + obs 128.
-BYTES_PER_ENV_FRAME = 222 + 1 + 222 + 128
+# action 1 + counters 28, write state 222 + obs 128 + counters 28 (counters:
+# cpu cycles, frame cycles, instructions as u64, status as u32).
+BYTES_PER_ENV_FRAME = (222 + 1 + 28) + (222 + 128 + 28)
 N_ENVS = 65536
+# VALU issue peak: 256 CUs x 4 SIMDs x 2.4 GHz / 4 clocks per wave64 VALU op.
+VALU_PEAK_G = 256 * 4 * 2.4 / 4
+# HBM bytes and VALU wave-instructions per launch measured with rocprofv3
+# --pmc for the default workload (profiles/*.md give the commands).
+PROFILE_FIGURES = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 
 
 def log(msg):
@@ -269,6 +275,12 @@ def main():
     gpu.kernel_time(reset=True)
 
     achieved_gbs = BYTES_PER_ENV_FRAME * n / (kernel_ms * 1e-3) / 1e9
+    measured = {}
+    if os.path.exists(PROFILE_FIGURES):
+        with open(PROFILE_FIGURES) as f:
+            measured = json.load(f).get(args.workload, {})
+        if measured.get("envs") != n:
+            measured = {}
     out = {
         "metric": "aggregate frames/sec at 64K envs (plain vs batched); p50/p99 served step latency",
         "value": round(plain_fps, 1),
@@ -293,8 +305,13 @@ def main():
         "instructions_per_frame": round(instr_total / frames_total, 1),
         "env_errors": status_bad,
         "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 6), "traffic": None},
+                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 6), "traffic": measured.get("traffic_bytes")},
     }
+    if "valu_wave_instr" in measured:
+        valu = measured["valu_wave_instr"] / (kernel_ms * 1e-3) / 1e9
+        out["valu_issue"] = {"achieved": round(valu, 2), "peak": VALU_PEAK_G, "unit": "G wave64-instr/s",
+                             "frac": round(valu / VALU_PEAK_G, 4),
+                             "note": "SQ_INSTS_VALU per launch (rocprofv3, profiles/) / live kernel time"}
     gpu.close()
     d_act.free()
     d_obs.free()

@@ -61,6 +61,26 @@ extern "C" {
     fn fa_download_states(env: *mut FaEnv, out: *mut AtariStateGpu, n: usize) -> c_int;
     fn fa_num_envs(env: *const FaEnv) -> usize;
     fn fa_last_error() -> *const c_char;
+    fn fa_upload_states(env: *mut FaEnv, states: *const AtariStateGpu, n: usize) -> c_int;
+    fn fa_download_counters(env: *mut FaEnv, cpu_cycles: *mut u64, frame_cycles: *mut u64,
+                            instructions: *mut u64, status: *mut u32) -> c_int;
+    fn fa_step_frameskip(env: *mut FaEnv, actions: *const u8, fs: *const u8, n: usize, obs_out: *mut u8) -> c_int;
+    fn fa_batch_create(roms: *const *const u8, rom_lens: *const usize, counts: *const usize, nclass: usize,
+                       out: *mut *mut FaBatch) -> c_int;
+    fn fa_batch_destroy(b: *mut FaBatch);
+    fn fa_batch_num_envs(b: *const FaBatch) -> usize;
+    fn fa_batch_reset(b: *mut FaBatch, obs_out: *mut u8) -> c_int;
+    fn fa_batch_step(b: *mut FaBatch, actions: *const u8, fs: *const u8, k: u32, obs_every: c_int,
+                     obs_out: *mut u8) -> c_int;
+    fn fa_batch_tune(b: *mut FaBatch, k: u32, use_fs: c_int, trials: *mut f64) -> c_int;
+    fn fa_batch_buffers(b: *mut FaBatch, k: u32, obs_every: c_int, d_actions: *mut *mut u8, d_fs: *mut *mut u8,
+                        d_obs: *mut *mut u8) -> c_int;
+    fn fa_memcpy_h2d(d_dst: *mut u8, h_src: *const u8, bytes: usize) -> c_int;
+}
+
+#[repr(C)]
+pub struct FaBatch {
+    _private: [u8; 0],
 }
 
 /// Error of a failed library call (the reference returns cudarc's `DriverError`).
@@ -128,8 +148,81 @@ impl BatchAtariGpu {
     }
 }
 
+// Extensions (no counterpart in src/cuda_env.rs).
+impl BatchAtariGpu {
+    /// Replace every env's state (checkpoint restore).
+    pub fn upload_states(&mut self, states: &[AtariStateGpu]) -> Result<(), HipError> {
+        check(unsafe { fa_upload_states(self.env, states.as_ptr(), states.len()) })
+    }
+
+    /// (cpu cycles, last-frame cycles, instructions, status) per env.
+    pub fn counters(&mut self) -> Result<(Vec<u64>, Vec<u64>, Vec<u64>, Vec<u32>), HipError> {
+        let (mut c, mut f, mut i, mut s) = (vec![0u64; self.n], vec![0u64; self.n], vec![0u64; self.n], vec![0u32; self.n]);
+        check(unsafe { fa_download_counters(self.env, c.as_mut_ptr(), f.as_mut_ptr(), i.as_mut_ptr(), s.as_mut_ptr()) })?;
+        Ok((c, f, i, s))
+    }
+
+    /// Env i holds actions[i] for fs[i] frames; obs after each env's last frame.
+    pub fn step_frameskip(&mut self, actions: &[u8], fs: &[u8]) -> Result<Vec<u8>, HipError> {
+        let mut obs = vec![0u8; self.n * 128];
+        check(unsafe { fa_step_frameskip(self.env, actions.as_ptr(), fs.as_ptr(), actions.len(), obs.as_mut_ptr()) })?;
+        Ok(obs)
+    }
+}
+
 impl Drop for BatchAtariGpu {
     fn drop(&mut self) {
         unsafe { fa_destroy(self.env) };
+    }
+}
+
+/// Several ROM classes stepped together by one hipGraph launch.
+pub struct BatchMixGpu {
+    b: *mut FaBatch,
+    n: usize,
+}
+
+unsafe impl Send for BatchMixGpu {}
+
+impl BatchMixGpu {
+    pub fn new(roms: &[Vec<u8>], counts: &[usize]) -> Result<Self, HipError> {
+        let ptrs: Vec<*const u8> = roms.iter().map(|r| r.as_ptr()).collect();
+        let lens: Vec<usize> = roms.iter().map(|r| r.len()).collect();
+        let mut b = std::ptr::null_mut();
+        check(unsafe { fa_batch_create(ptrs.as_ptr(), lens.as_ptr(), counts.as_ptr(), roms.len(), &mut b) })?;
+        let n = unsafe { fa_batch_num_envs(b) };
+        Ok(Self { b, n })
+    }
+
+    pub fn reset(&mut self) -> Result<Vec<u8>, HipError> {
+        let mut obs = vec![0u8; self.n * 128];
+        check(unsafe { fa_batch_reset(self.b, obs.as_mut_ptr()) })?;
+        Ok(obs)
+    }
+
+    /// k steps: actions / fs laid out [s * N + i]; obs after the last step.
+    pub fn step(&mut self, actions: &[u8], fs: Option<&[u8]>, k: u32) -> Result<Vec<u8>, HipError> {
+        assert_eq!(actions.len(), k as usize * self.n);
+        let mut obs = vec![0u8; self.n * 128];
+        let fsp = fs.map_or(std::ptr::null(), |f| f.as_ptr());
+        check(unsafe { fa_batch_step(self.b, actions.as_ptr(), fsp, k, 0, obs.as_mut_ptr()) })?;
+        Ok(obs)
+    }
+
+    /// Autotune block size / grouping / steps per launch per ROM class.
+    pub fn tune(&mut self, actions: &[u8], fs: Option<&[u8]>, k: u32) -> Result<(), HipError> {
+        let (mut da, mut df, mut d_obs) = (std::ptr::null_mut(), std::ptr::null_mut(), std::ptr::null_mut());
+        check(unsafe { fa_batch_buffers(self.b, k, 0, &mut da, &mut df, &mut d_obs) })?;
+        check(unsafe { fa_memcpy_h2d(da, actions.as_ptr(), actions.len()) })?;
+        if let Some(f) = fs {
+            check(unsafe { fa_memcpy_h2d(df, f.as_ptr(), f.len()) })?;
+        }
+        check(unsafe { fa_batch_tune(self.b, k, fs.is_some() as c_int, std::ptr::null_mut()) })
+    }
+}
+
+impl Drop for BatchMixGpu {
+    fn drop(&mut self) {
+        unsafe { fa_batch_destroy(self.b) };
     }
 }

@@ -24,6 +24,12 @@
  *     capped at ORC_LOOPA_CAP cycles, after which `error = ORC_ERR_VSYNC_HELD`.
  * An env with a non-zero error is frozen: later frames do nothing.
  *
+ * Parity unpinned (stated only by the reference's text, restated here and
+ * reviewed, no independent check): the HeadlessBus flat-array read quirk --
+ * RAM-mirror reads outside $80-$FF / $180-$1FF return 0 (src/atari.rs:386-395,
+ * 503-507) -- and the frame-loop caps (src/atari.rs:594).  Everything else is
+ * pinned by tests/test_oracle_*.py and tests/golden/.
+ *
  * Licence note: the reference is GPL-3.0; this restatement follows its
  * algorithm and is kept under the same licence terms.
  */
