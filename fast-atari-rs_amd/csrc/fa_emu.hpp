@@ -753,9 +753,15 @@ struct Emu {
         uint64_t rem = __ballot(1);
         do {
             const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)__builtin_ctzll(rem));
-            const bool mine = key == lk;
+            // The lane test compares against an opaque copy of lk: were it
+            // lk itself, the optimiser would substitute the per-lane key for
+            // lk where the two are equal, and the dispatch below would become
+            // a VALU compare tree under exec masks instead of scalar branches.
+            uint32_t lk_cmp = lk;
+            __asm__ volatile("" : "+s"(lk_cmp));
+            const bool mine = key == lk_cmp;
             const uint64_t grp = __ballot(mine);
-            if ((GROUP || grp == rem) && mine && lk < 0x100u) {
+            if (lk < 0x100u && (GROUP ? mine : grp == rem)) {
                 switch (lk) {
 #define FA_CASE(OPC)                                                                          \
                 case OPC: {                                                                   \
