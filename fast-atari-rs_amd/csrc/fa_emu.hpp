@@ -85,6 +85,64 @@ __device__ __forceinline__ void st16(uint32_t off, uint32_t v) {
 __device__ __forceinline__ uint32_t mk(bool c) { return 0u - (uint32_t)c; }
 __device__ __forceinline__ uint32_t pick(bool c, uint32_t t, uint32_t f) { return (t & mk(c)) | (f & ~mk(c)); }
 
+// ---------------------------------------------------------------- RIOT timer
+// Pia::tick_n (src/pia.rs:107-166) with the prescaler division done by shifts
+// (select is 1, 8, 64 or 1024; tsh = log2(select)).  T is any holder of the
+// timer fields: the per-lane Emu, or the wave-uniform control state.
+template <class T>
+__device__ __forceinline__ void timer_tick_slow(T &t, uint32_t n) {
+    if (n == 0) return;
+    if (t.tval == 0 && t.tuf) {
+        t.tval = (0x100u - n) & 0xFFu;
+        return;
+    }
+    uint32_t rem = n;
+    if (t.tps > 0) {
+        if (rem < t.tps) { t.tps -= rem; return; }
+        rem -= t.tps;
+        t.tps = 0;
+        if (t.tval == 0) {
+            t.tuf = 1;
+            t.tval = rem ? ((0x100u - rem) & 0xFFu) : 0xFFu;
+            return;
+        }
+        t.tval -= 1;
+    }
+    if (rem == 0) { t.tps = t.tsel; return; }
+    const uint32_t full = rem >> t.tsh;
+    const uint32_t left = rem & (t.tsel - 1u);
+    if (full >= t.tval) {
+        rem -= t.tval << t.tsh;
+        t.tuf = 1;
+        t.tval = rem ? ((0x100u - rem) & 0xFFu) : 0xFFu;
+    } else {
+        t.tval -= full;
+        t.tps = t.tsel - left;
+    }
+}
+template <class T>
+__device__ __forceinline__ void timer_tick(T &t, uint32_t n) {
+    const bool post = t.tval == 0 && t.tuf;
+    if (!post && t.tps > n) { t.tps -= n; return; }       // common case: inside a prescaler period
+    timer_tick_slow(t, n);
+}
+// Timer write (Pia::write with addr & 0x10, src/pia.rs:202-212).
+template <class T>
+__device__ __forceinline__ void timer_set(T &t, uint32_t addr, uint32_t v) {
+    t.tval = v;
+    t.tuf = 0;
+    t.tsh = (0x0A060300u >> ((addr & 3u) * 8u)) & 0xFFu;   // 1, 8, 64, 1024
+    t.tsel = 1u << t.tsh;
+    t.tps = t.tsel;
+}
+
+// (clock - 68 + 5) rem_euclid 160 for clock < 228 (src/headless_tia.rs:257-261)
+__device__ __forceinline__ uint32_t resp_pos_of(uint32_t clock) {
+    uint32_t t = clock + 97u;
+    t = t >= 160u ? t - 160u : t;
+    return t >= 160u ? t - 160u : t;
+}
+
 struct Emu {
     // CPU
     uint32_t a, x, y, sp, p, pc;
@@ -136,43 +194,7 @@ struct Emu {
     }
 
     // ------------------------------------------------------------ RIOT
-    // Pia::tick_n (src/pia.rs:107-166) with the prescaler division done by
-    // shifts (select is 1, 8, 64 or 1024).
-    __device__ __forceinline__ void pia_tick(uint32_t n) {
-        const bool post = tval == 0 && tuf;
-        if (!post && tps > n) { tps -= n; return; }       // common case: inside a prescaler period
-        pia_tick_slow(n);
-    }
-    __device__ __forceinline__ void pia_tick_slow(uint32_t n) {
-        if (n == 0) return;
-        if (tval == 0 && tuf) {
-            tval = (0x100u - n) & 0xFFu;
-            return;
-        }
-        uint32_t rem = n;
-        if (tps > 0) {
-            if (rem < tps) { tps -= rem; return; }
-            rem -= tps;
-            tps = 0;
-            if (tval == 0) {
-                tuf = 1;
-                tval = rem ? ((0x100u - rem) & 0xFFu) : 0xFFu;
-                return;
-            }
-            tval -= 1;
-        }
-        if (rem == 0) { tps = tsel; return; }
-        uint32_t full = rem >> tsh;
-        uint32_t left = rem & (tsel - 1u);
-        if (full >= tval) {
-            rem -= tval << tsh;
-            tuf = 1;
-            tval = rem ? ((0x100u - rem) & 0xFFu) : 0xFFu;
-        } else {
-            tval -= full;
-            tps = tsel - left;
-        }
-    }
+    __device__ __forceinline__ void pia_tick(uint32_t n) { timer_tick(*this, n); }
 
     // Pia::read for I/O addresses (src/pia.rs:169-199)
     __device__ __forceinline__ uint32_t pia_read(uint32_t addr) {
@@ -190,11 +212,7 @@ struct Emu {
     // Pia::write for I/O addresses (src/pia.rs:202-228)
     __device__ __forceinline__ void pia_write(uint32_t addr, uint32_t v) {
         if (addr & 0x10u) {
-            tval = v;
-            tuf = 0;
-            tsh = (0x0A060300u >> ((addr & 3u) * 8u)) & 0xFFu;   // 1, 8, 64, 1024
-            tsel = 1u << tsh;
-            tps = tsel;
+            timer_set(*this, addr, v);
             return;
         }
         // SWCHA, SWACNT, SWCHB, SWBCNT -> port_a_output, port_a_ddr, port_b_output, port_b_ddr
@@ -241,12 +259,6 @@ struct Emu {
         return 0u;
     }
 
-    // (clock - 68 + 5) rem_euclid 160 for clock < 228 (src/headless_tia.rs:257-261)
-    __device__ __forceinline__ uint32_t resp_pos() const {
-        uint32_t t = clock + 97u;
-        t = t >= 160u ? t - 160u : t;
-        return t >= 160u ? t - 160u : t;
-    }
 
     // HeadlessTia::write (src/headless_tia.rs:225-298)
     __device__ __forceinline__ void tia_write(uint32_t addr, uint32_t v) {
@@ -268,7 +280,7 @@ struct Emu {
         case 0x0B: case 0x0C: st8(img + S_REFP0 + (r - 0x0Bu), (v >> 3) & 1u); break;
         case 0x0D: case 0x0E: case 0x0F: st8(img + S_PF0 + (r - 0x0Du), v); break;
         case 0x10: case 0x11: case 0x12: case 0x13: case 0x14:
-            st16(img + S_POS + 2u * (r - 0x10u), resp_pos());
+            st16(img + S_POS + 2u * (r - 0x10u), resp_pos_of(clock));
             break;
         case 0x1B:
             st8(img + S_GRP0_OLD, ld8(img + S_GRP0));

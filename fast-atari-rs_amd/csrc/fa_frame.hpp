@@ -1,0 +1,201 @@
+// fa_frame.hpp -- the frame-step kernel (template) shared by the two
+// translation units that instantiate it: fa_kernels.hip (the plain path) and
+// fa_kernels_sorted.hip (wave64 opcode grouping).  They are compiled with
+// different code-generation options; see fastatari/build.py.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fa_emu.hpp"
+#include "fa_internal.hpp"
+#include "fa_uniform.hpp"
+
+namespace fa {
+
+// HeadlessAtari::run_frame (src/atari.rs:584-614).  Returns the frame's
+// cycle count; sets `st` on an illegal opcode or a held VSYNC.
+//
+// Whenever the active lanes agree on the whole control state the wave runs the
+// loop in the wave-uniform mode (fa_uniform.hpp) until the frame ends or an
+// instruction would make the lanes disagree; the per-lane loop below runs that
+// instruction and everything else.  Before each per-lane instruction a cheap
+// test (PC and TIA clock) decides whether the full test is worth making; a
+// failed full test backs off for a growing number of instructions.
+template <bool SORTED>
+__device__ __forceinline__ uint32_t run_frame(Emu &e, uint32_t &st, uint32_t &cpu_cyc, uint32_t &instr) {
+    e.fcomp = 0;
+    uint32_t cycles = 0;
+    bool in_a = (e.vsync & 2u) != 0u;
+    uint32_t hold = 0, backoff = 0;      // wave-uniform
+    while (true) {
+        if (hold == 0u) {
+            if (u_maybe(e)) {
+                UCtl u;
+                if (u_enter(e, cycles, in_a, u)) {
+                    const uint32_t c0 = u.cycles;
+                    const uint32_t r = u_run(u, e, st);
+                    const bool moved = u.cycles != c0;
+                    u_leave(u, e, cycles, in_a, cpu_cyc, instr);
+                    if (r == U_DONE) break;
+                    // declined at once: back off as after a failed test
+                    backoff = moved ? 0u : (backoff ? (backoff < 64u ? 2u * backoff : 64u) : 1u);
+                    hold = backoff;
+                } else {
+                    backoff = backoff ? (backoff < 64u ? 2u * backoff : 64u) : 1u;
+                    hold = backoff;
+                }
+            }
+        } else {
+            --hold;
+        }
+        if (in_a && !(e.vsync & 2u)) in_a = false;
+        if (!in_a && (e.vsync & 2u)) break;
+        if (e.wsync) {
+            uint32_t s = e.tia_skip();
+            if (s == 0u) { st = ENV_VSYNC_HELD; break; }   // only a clock >= 228 could stall here
+            e.pia_tick(s);
+            cycles += s;
+        } else {
+            uint32_t c = e.exec<SORTED>();
+            if (c == 0u) { st = ENV_ILLEGAL; break; }
+            cycles += c;
+            cpu_cyc += c;
+            instr += 1u;
+            e.tia_tick(c);
+            e.pia_tick(c);
+        }
+        if (in_a) {
+            if (cycles > LOOPA_CAP) { st = ENV_VSYNC_HELD; break; }
+        } else if (cycles > LOOPB_CAP) {
+            break;
+        }
+    }
+    return cycles;
+}
+
+// RAM bytes of an LDS image (2-byte aligned) to 128 bytes of global memory.
+__device__ __forceinline__ void put_obs(uint8_t *dst, uint32_t im) {
+    uint4 *d = reinterpret_cast<uint4 *>(dst);
+    for (uint32_t part = 0; part < 8u; ++part) {
+        const uint32_t off = im + S_RAM + part * 16u;
+        uint32_t w[4];
+        for (int j = 0; j < 4; ++j) w[j] = ld16(off + 4u * j) | (ld16(off + 4u * j + 2u) << 16);
+        d[part] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+
+template <bool SORTED>
+__global__ void __launch_bounds__(256) fa_frame_kernel(StepArgs g) {
+    const uint32_t bs = blockDim.x;
+    const uint32_t tid = threadIdx.x;
+    uint32_t first, count;
+    const uint32_t *pk_in = nullptr;   // packed inputs of this workgroup's envs
+    uint8_t *obs_out = nullptr;        // obs of this workgroup's envs (last step)
+    if (g.htasks) {
+        const HostTask t = g.htasks[blockIdx.x];
+        first = t.first;
+        count = t.count;
+        pk_in = t.in;
+        obs_out = t.obs;
+    } else if (g.tasks) {
+        first = g.tasks[2u * blockIdx.x];
+        count = g.tasks[2u * blockIdx.x + 1u];
+    } else {
+        first = blockIdx.x * bs;
+        count = g.n > first ? (g.n - first < bs ? g.n - first : bs) : 0u;
+    }
+    if (!g.htasks) {
+        if (g.packed) pk_in = g.packed + first;
+        if (g.obs && !g.obs_every) obs_out = g.obs + (size_t)first * 128u;
+    }
+    // The AoS block is staged in whole 16-byte granules: 8 envs = 1776 bytes.
+    const uint32_t staged = (count + 7u) & ~7u;
+    const uint32_t img0 = ROM0 + g.rom_bytes;
+    const uint32_t img_bytes = staged * STATE_BYTES;
+    const uint32_t junk0 = img0 + bs * STATE_BYTES;
+
+    // Stage the descriptors, the ROM and this workgroup's state block into
+    // LDS (16-byte loads).
+    {
+        const uint4 *dsrc = reinterpret_cast<const uint4 *>(g.desc);
+        uint4 *ddst = reinterpret_cast<uint4 *>(fa_lds + DESC0);
+        for (uint32_t i = tid; i < 2048u / 16u; i += bs) ddst[i] = dsrc[i];
+        const uint4 *src = reinterpret_cast<const uint4 *>(g.rom);
+        uint4 *dst = reinterpret_cast<uint4 *>(fa_lds + ROM0);
+        for (uint32_t i = tid; i < g.rom_bytes / 16u; i += bs) dst[i] = src[i];
+        const uint4 *ssrc = reinterpret_cast<const uint4 *>(g.states + (size_t)first * STATE_BYTES);
+        uint4 *sdst = reinterpret_cast<uint4 *>(fa_lds + img0);
+        for (uint32_t i = tid; i < img_bytes / 16u; i += bs) sdst[i] = ssrc[i];
+    }
+    __syncthreads();
+
+    if (tid < count) {
+        const uint32_t idx = first + tid;
+        const uint32_t im = img0 + tid * STATE_BYTES;
+        uint64_t cpu0, fcyc0, instr0;
+        uint32_t st;
+        const uint32_t pk = pk_in ? pk_in[tid] : 0u;
+        const bool restart = pk_in ? ((pk >> 16) & 0xFFu) != 0u : (g.reset && g.reset[idx]);
+        if (restart) {
+            // restart: the ROM's reset snapshot and its counters
+            const uint16_t *src = reinterpret_cast<const uint16_t *>(g.reset_state);
+            for (uint32_t j = 0; j < STATE_BYTES / 2u; ++j) st16(im + 2u * j, src[j]);
+            cpu0 = g.reset_counters[0];
+            fcyc0 = g.reset_counters[1];
+            instr0 = g.reset_counters[2];
+            st = ENV_OK;
+        } else {
+            cpu0 = g.cpu_cycles[idx];
+            fcyc0 = g.frame_cycles[idx];
+            instr0 = g.instr[idx];
+            st = g.status[idx];
+        }
+        Emu e;
+        e.load(im, junk0 + tid, g.hot_lo, g.hot_n);
+        uint32_t cpu_cyc = 0, instr = 0;
+        uint64_t fcyc = fcyc0;
+        for (uint32_t s = 0; s < g.k; ++s) {
+            const size_t at = (size_t)s * g.stride + idx;
+            uint32_t nfr;
+            if (pk_in) {
+                if (!(pk & (1u << 24))) e.set_action(pk & 0xFFu);
+                nfr = (pk >> 8) & 0xFFu;
+            } else {
+                if (g.actions) e.set_action(g.actions[at]);
+                nfr = g.fs ? (uint32_t)g.fs[at] : 1u;
+            }
+            for (uint32_t f = 0; f < nfr; ++f) {
+                if (st != ENV_OK) { fcyc = 0; continue; }
+                fcyc = run_frame<SORTED>(e, st, cpu_cyc, instr);
+            }
+            if (g.obs && g.obs_every) {
+                e.store();
+                put_obs(g.obs + at * 128u, im);
+            }
+        }
+        e.store();
+        g.cpu_cycles[idx] = cpu0 + cpu_cyc;
+        g.frame_cycles[idx] = fcyc;
+        g.instr[idx] = instr0 + instr;
+        g.status[idx] = st;
+    }
+    __syncthreads();
+
+    {
+        uint4 *sdst = reinterpret_cast<uint4 *>(g.states + (size_t)first * STATE_BYTES);
+        const uint4 *ssrc = reinterpret_cast<const uint4 *>(fa_lds + img0);
+        for (uint32_t i = tid; i < img_bytes / 16u; i += bs) sdst[i] = ssrc[i];
+    }
+    if (obs_out) {
+        // obs of env j = RAM bytes 8..135 of its image; 8 lanes per env
+        for (uint32_t c = tid; c < count * 8u; c += bs) {
+            const uint32_t env = c >> 3, part = c & 7u;
+            const uint32_t off = img0 + env * STATE_BYTES + S_RAM + part * 16u;
+            uint32_t w[4];
+            for (int j = 0; j < 4; ++j) w[j] = ld16(off + 4u * j) | (ld16(off + 4u * j + 2u) << 16);
+            reinterpret_cast<uint4 *>(obs_out + (size_t)env * 128u)[part] = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+    }
+}
+
+}  // namespace fa

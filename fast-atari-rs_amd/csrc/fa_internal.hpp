@@ -75,6 +75,9 @@ uint32_t default_block(uint32_t rom_bytes, size_t max_lds);
 // Kernel launchers (fa_kernels.hip).
 hipError_t prepare_kernels(size_t max_lds);
 hipError_t launch_step(const StepArgs &g, bool sorted, uint32_t bs, uint32_t grid, hipStream_t s);
+// The opcode-grouping kernel lives in its own translation unit (fa_kernels_sorted.hip).
+hipError_t prepare_sorted_kernel(size_t max_lds);
+hipError_t launch_sorted(const StepArgs &g, uint32_t bs, uint32_t grid, size_t lds, hipStream_t s);
 hipError_t launch_fill(const AtariStateGpu &tmpl, uint8_t *states, uint64_t *cpu, uint64_t *fcyc, uint64_t *instr,
                        uint32_t *status, uint32_t n, uint32_t n_pad, hipStream_t s);
 void descriptor_table(uint32_t out[512]);
