@@ -109,7 +109,7 @@ __device__ __forceinline__ void u_end_scanline(UCtl &u, Emu &e) {
 }
 __device__ __forceinline__ void u_tia_tick(UCtl &u, Emu &e, uint32_t n) {
     u.clock += 3u * n;
-    if (u.clock >= 228u) u_end_scanline(u, e);
+    if (__builtin_expect(u.clock >= 228u, 0)) u_end_scanline(u, e);
 }
 __device__ __forceinline__ uint32_t u_tia_skip(UCtl &u, Emu &e) {
     const uint32_t rem = u.clock >= 228u ? 0u : 228u - u.clock;
@@ -469,18 +469,20 @@ __device__ __forceinline__ bool u_special(UCtl &u, Emu &e, uint32_t b1, uint32_t
     return true;
 }
 
-// One instruction with its descriptor known at compile time; false (nothing
-// changed) when it needs the per-lane path.
+// One instruction with its descriptor known at compile time.  Returns its
+// cycle count, or 0 (nothing changed) when it needs the per-lane path.
 template <uint64_t D>
-__device__ __forceinline__ bool u_op(UCtl &u, Emu &e, uint32_t b1, uint32_t b2, uint32_t &cyc) {
+__device__ __forceinline__ uint32_t u_op(UCtl &u, Emu &e, uint32_t b1, uint32_t b2) {
     constexpr uint32_t D0 = (uint32_t)D, D1 = (uint32_t)(D >> 32);
     constexpr uint32_t cls = D0 & 3u;
     const uint32_t pcn = (u.pc + ((D0 >> D_LEN_SHIFT) & 3u)) & 0xFFFFu;
-    cyc = (D0 >> D_CYC_SHIFT) & 7u;
-    if constexpr (cls == CLS_ILLEGAL) return false;
-    else if constexpr (cls == CLS_BRANCH) return u_branch<D1>(u, e, b1, pcn, cyc);
-    else if constexpr (cls == CLS_SPECIAL) return u_special<D1>(u, e, b1, b2, pcn);
-    else return u_compute<D0, D1>(u, e, b1, b2, pcn, cyc);
+    uint32_t cyc = (D0 >> D_CYC_SHIFT) & 7u;
+    bool ok;
+    if constexpr (cls == CLS_ILLEGAL) ok = false;
+    else if constexpr (cls == CLS_BRANCH) ok = u_branch<D1>(u, e, b1, pcn, cyc);
+    else if constexpr (cls == CLS_SPECIAL) ok = u_special<D1>(u, e, b1, b2, pcn);
+    else ok = u_compute<D0, D1>(u, e, b1, b2, pcn, cyc);
+    return ok ? cyc : 0u;
 }
 
 // ---------------------------------------------------------------- frame loop
@@ -500,8 +502,13 @@ __device__ __forceinline__ uint32_t u_run(UCtl &u, Emu &e, uint32_t &st) {
     u.cpu = 0;
     u.instr = 0;
     while (true) {
-        if (u.in_a && !(u.vsync & 2u)) u.in_a = 0;
-        if (!u.in_a && (u.vsync & 2u)) return U_DONE;
+        // loop A runs while VSYNC is held and leaves when it drops; loop B
+        // ends the frame when VSYNC rises
+        const uint32_t v2 = (u.vsync >> 1) & 1u;
+        if (__builtin_expect(u.in_a != v2, 0)) {
+            if (!u.in_a) return U_DONE;
+            u.in_a = 0;
+        }
         if (u.wsync) {
             const uint32_t s = u_tia_skip(u, e);
             if (s == 0u) { st = ENV_VSYNC_HELD; return U_DONE; }
@@ -510,31 +517,33 @@ __device__ __forceinline__ uint32_t u_run(UCtl &u, Emu &e, uint32_t &st) {
         } else {
             // opcode and operands straight from the LDS ROM image: below the
             // hotspot / vector area these fetches have no side effects
-            if (!(u.pc & 0x1000u) || (u.pc & 0xFFFu) >= 0xFF0u) return U_EXIT;
+            if (__builtin_expect(((u.pc & 0x1FFFu) - 0x1000u) >= 0xFF0u, 0)) return U_EXIT;
             const uint32_t o = ROM0 + (u.bank << 12) + (u.pc & 0xFFFu);
             const uint32_t op = first_lane(ld8(o)), b1 = first_lane(ld8(o + 1u)), b2 = first_lane(ld8(o + 2u));
             uint32_t cyc = 0;
-            bool ok = false;
             switch (op) {
 #define FA_UCASE(OPC)                                                   \
             case OPC:                                                   \
-                ok = u_op<descriptor_of(OPC)>(u, e, b1, b2, cyc);       \
+                cyc = u_op<descriptor_of(OPC)>(u, e, b1, b2);           \
                 break;
                 FA_FOR_EACH_OPCODE(FA_UCASE)
 #undef FA_UCASE
             default: break;
             }
-            if (!ok) return U_EXIT;
+            if (__builtin_expect(cyc == 0u, 0)) return U_EXIT;
             u.cycles += cyc;
             u.cpu += cyc;
             u.instr += 1u;
             u_tia_tick(u, e, cyc);
-            timer_tick(u, cyc);
+            // Pia::tick_n's fast branch inline, the rest out of line
+            if (__builtin_expect(u.tps > cyc, 1) && (u.tval != 0u || !u.tuf)) u.tps -= cyc;
+            else timer_tick_slow(u, cyc);
         }
-        if (u.in_a) {
+        // the frame-loop caps (loop A's is the larger)
+        static_assert(LOOPA_CAP > LOOPB_CAP, "cap order");
+        if (__builtin_expect(u.cycles > LOOPB_CAP, 0)) {
+            if (!u.in_a) return U_DONE;
             if (u.cycles > LOOPA_CAP) { st = ENV_VSYNC_HELD; return U_DONE; }
-        } else if (u.cycles > LOOPB_CAP) {
-            return U_DONE;
         }
     }
 }
