@@ -501,6 +501,14 @@ __device__ __forceinline__ uint32_t u_run(UCtl &u, Emu &e, uint32_t &st) {
     u.cycles = wave_first(u.cycles); u.in_a = wave_first(u.in_a);
     u.cpu = 0;
     u.instr = 0;
+    // opcode and operand bytes at PC, loaded one iteration ahead so that the
+    // LDS latency overlaps the previous instruction's timing work (the address
+    // stays inside the LDS allocation for any PC and bank)
+    uint32_t f0, f1, f2;
+    {
+        const uint32_t o = ROM0 + (u.bank << 12) + (u.pc & 0xFFFu);
+        f0 = ld8(o); f1 = ld8(o + 1u); f2 = ld8(o + 2u);
+    }
     while (true) {
         // loop A runs while VSYNC is held and leaves when it drops; loop B
         // ends the frame when VSYNC rises
@@ -518,8 +526,7 @@ __device__ __forceinline__ uint32_t u_run(UCtl &u, Emu &e, uint32_t &st) {
             // opcode and operands straight from the LDS ROM image: below the
             // hotspot / vector area these fetches have no side effects
             if (__builtin_expect(((u.pc & 0x1FFFu) - 0x1000u) >= 0xFF0u, 0)) return U_EXIT;
-            const uint32_t o = ROM0 + (u.bank << 12) + (u.pc & 0xFFFu);
-            const uint32_t op = first_lane(ld8(o)), b1 = first_lane(ld8(o + 1u)), b2 = first_lane(ld8(o + 2u));
+            const uint32_t op = first_lane(f0), b1 = first_lane(f1), b2 = first_lane(f2);
             uint32_t cyc = 0;
             switch (op) {
 #define FA_UCASE(OPC)                                                   \
@@ -538,6 +545,8 @@ __device__ __forceinline__ uint32_t u_run(UCtl &u, Emu &e, uint32_t &st) {
             // Pia::tick_n's fast branch inline, the rest out of line
             if (__builtin_expect(u.tps > cyc, 1) && (u.tval != 0u || !u.tuf)) u.tps -= cyc;
             else timer_tick_slow(u, cyc);
+            const uint32_t o = ROM0 + (u.bank << 12) + (u.pc & 0xFFFu);
+            f0 = ld8(o); f1 = ld8(o + 1u); f2 = ld8(o + 2u);
         }
         // the frame-loop caps (loop A's is the larger)
         static_assert(LOOPA_CAP > LOOPB_CAP, "cap order");
