@@ -312,6 +312,13 @@ def main():
         out["valu_issue"] = {"achieved": round(valu, 2), "peak": VALU_PEAK_G, "unit": "G wave64-instr/s",
                              "frac": round(valu / VALU_PEAK_G, 4),
                              "note": "SQ_INSTS_VALU per launch (rocprofv3, profiles/) / live kernel time"}
+    if "wave_instr" in measured:
+        # every wave-instruction (VALU, SALU, branch, LDS, SMEM) against one
+        # issue per SIMD every 4 clocks: the bound of a SIMD's single wave
+        iss = measured["wave_instr"] / (kernel_ms * 1e-3) / 1e9
+        out["issue"] = {"achieved": round(iss, 2), "peak": VALU_PEAK_G, "unit": "G wave-instr/s",
+                        "frac": round(iss / VALU_PEAK_G, 4),
+                        "note": "SQ_INSTS_{VALU,SALU,BRANCH,LDS,SMEM} per launch (rocprofv3, profiles/) / live kernel time"}
     gpu.close()
     d_act.free()
     d_obs.free()
@@ -324,7 +331,8 @@ def main():
         ol = out["served"]["open_loop"]
         out["served_p50_ms"], out["served_p99_ms"] = ol["p50_ms"], ol["p99_ms"]
     if not args.no_cpu:
-        sample_n, sample_frames = 8192, 48
+        # a bounded sample of the same workload, about 10 s on 16 host threads
+        sample_n, sample_frames = 16384, 192
         cpu_fps, secs = cpu_baseline(rom, sample_n, sample_frames, seed, args.cpu_threads)
         out["cpu_baseline"] = {"value": round(cpu_fps, 1), "unit": "frames/s", "cores": args.cpu_threads,
                                "kind": "port",
