@@ -22,6 +22,12 @@ import os
 import sys
 import time
 
+# The served mode keeps one stream per ROM class plus an admin stream; the
+# HIP runtime maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 unless
+# set) and serialises the kernels of streams that share one.  Set before
+# anything initialises HIP.
+os.environ["GPU_MAX_HW_QUEUES"] = "8"
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for _p in (ROOT, os.path.join(ROOT, "fast-atari-rs_amd")):
     if _p not in sys.path:

@@ -50,7 +50,14 @@ constexpr uint32_t kChunkSlots = 65536;
 constexpr uint32_t kUnit = 64;                 // slot allocation granule
 constexpr uint32_t kUnits = kChunkSlots / kUnit;
 constexpr uint32_t kCtxPerClass = 1;
-constexpr uint32_t kTaskEnvs = 256;            // envs per workgroup
+constexpr uint32_t kTaskEnvs = 256;            // envs per workgroup of the admin launches
+// A class's workgroup size: the largest of 256 / 128 / 64 envs whose LDS
+// footprint stays within kShareLds, so that workgroups of two classes fit on
+// one CU together (160 KB) -- a slow class's long-running workgroups then do
+// not keep another class's launches waiting for a free CU.  (Smaller
+// workgroups than needed spread a slow class's waves over more CUs, where they
+// share SIMDs with the other classes' waves.)
+constexpr size_t kShareLds = 80u << 10;
 constexpr uint32_t kMaxClientEnvs = kChunkSlots;
 constexpr size_t kSlotBytes = fa::kStateBytes + 3 * 8 + 4 + 4 + 128;
 
@@ -200,6 +207,14 @@ fa::StepArgs chunk_args(const fa_server *s, const ServerClass &sc, const Chunk &
     return g;
 }
 
+uint32_t serve_block(uint32_t rom_bytes, size_t max_lds) {
+    for (uint32_t b : {256u, 128u}) {
+        const size_t l = fa::lds_bytes(rom_bytes, b);
+        if (l <= kShareLds && l <= max_lds) return b;
+    }
+    return 64u;
+}
+
 uint32_t fill_tasks(uint32_t *tasks, uint32_t first, uint32_t n) {
     uint32_t t = 0;
     for (uint32_t off = 0; off < n; off += kTaskEnvs) {
@@ -263,6 +278,8 @@ void dispatch_loop(fa_server *s) {
         ServerClass &sc = s->cls[pick];
         // every queued request on the head request's chunk
         const int chunk = s->clients[sc.queue.front()].chunk;
+        const uint32_t bs = sc.block;
+        const int sorted = sc.sorted;
         ctx->clients.clear();
         std::deque<int> rest;
         uint32_t ntasks = 0;
@@ -270,7 +287,7 @@ void dispatch_loop(fa_server *s) {
             Client &cl = s->clients[id];
             if (cl.chunk == chunk) {
                 ctx->clients.push_back(id);
-                ntasks += (cl.n + kTaskEnvs - 1) / kTaskEnvs;
+                ntasks += (cl.n + bs - 1) / bs;
                 cl.state = 2;
             } else {
                 rest.push_back(id);
@@ -301,19 +318,19 @@ void dispatch_loop(fa_server *s) {
                 ctx->task_cap = cap;
         }
         if (rc == FA_OK) {
-            // one workgroup per <= 256 envs of each client; inputs and obs stay in the client's pinned buffers
+            // one workgroup per <= bs envs of each client; inputs and obs stay in the client's pinned buffers
             uint32_t t = 0;
             for (int id : ctx->clients) {
                 const Client &cl = s->clients[id];
-                for (uint32_t off = 0; off < cl.n; off += kTaskEnvs)
-                    ctx->h_tasks[t++] = fa::HostTask{cl.first + off, std::min(kTaskEnvs, cl.n - off), cl.dv_in + off,
+                for (uint32_t off = 0; off < cl.n; off += bs)
+                    ctx->h_tasks[t++] = fa::HostTask{cl.first + off, std::min(bs, cl.n - off), cl.dv_in + off,
                                                      cl.dv_obs + (size_t)off * 128u};
             }
             fa::StepArgs g = chunk_args(s, sc, ch);
             g.packed = nullptr;
             g.obs = nullptr;
             g.htasks = ctx->dv_tasks;
-            hip_ok(fa::launch_step(g, sc.sorted != 0, sc.block, t, ctx->stream), "frame kernel");
+            hip_ok(fa::launch_step(g, sorted != 0, bs, t, ctx->stream), "frame kernel");
             hip_ok(hipEventRecord(ctx->done, ctx->stream), "hipEventRecord");
         }
         lk.lock();
@@ -501,7 +518,7 @@ int fa_server_create(const uint8_t *const *roms, const size_t *rom_lens, size_t 
             }
             sc.ctx.push_back(std::move(ctx));
         }
-        sc.block = kTaskEnvs;
+        sc.block = serve_block(sc.rom.rom_bytes, s->max_lds);
     }
     s->dispatcher = std::thread(dispatch_loop, s);
     for (size_t c = 0; c < nclass; ++c) s->completers.emplace_back(complete_loop, s, c);
@@ -549,8 +566,13 @@ void fa_server_destroy(fa_server *s) {
 int fa_server_set_class_config(fa_server *s, size_t c, uint32_t block, int sorted) {
     if (!s) return fail(FA_EINVAL, "NULL server");
     if (c >= s->cls.size()) return fail(FA_EINVAL, "class index out of range");
-    if (block != kTaskEnvs) return fail(FA_EINVAL, "server launches use 256-env workgroups");
+    if (block != 0 && block != 64 && block != 128 && block != 256)
+        return fail(FA_EINVAL, "block must be 0 (the default), 64, 128 or 256");
+    const uint32_t bs = block ? block : serve_block(s->cls[c].rom.rom_bytes, s->max_lds);
+    if (fa::lds_bytes(s->cls[c].rom.rom_bytes, bs) > s->max_lds)
+        return fail(FA_EINVAL, "block too large for the LDS budget with this ROM");
     std::lock_guard<std::mutex> lk(s->mu);
+    s->cls[c].block = bs;
     s->cls[c].sorted = sorted != 0;
     return FA_OK;
 }

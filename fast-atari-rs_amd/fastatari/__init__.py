@@ -10,6 +10,15 @@ built library or without a GPU raises.
 launch shapes from its autotuner); `EnvServer` serves step requests of many
 clients with continuous batching.
 """
+import os as _os
+
+# The HIP runtime maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 by
+# default), and kernels of streams that share a queue run one after another.
+# The server and the batch keep one stream per ROM class plus one more, so a
+# process that has not initialised HIP yet gets 8 queues; an explicit setting
+# is left as it is.
+_os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+
 from .env import (STATE_DTYPE, BatchAtariGpu, Busy, FastAtariError, DeviceBuffer, lib_path, load_library)
 from .batch import BatchMixGpu
 from .serve import EnvServer

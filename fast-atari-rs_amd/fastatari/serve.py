@@ -99,8 +99,10 @@ class EnvServer:
         return {"states": st.view(np.uint8).reshape(n, 222), "cpu_cycles": cyc, "frame_cycles": fc,
                 "status": status}
 
-    def set_class_config(self, cls: int, sorted_: bool = False):
-        _check(load_library().fa_server_set_class_config(self._h, int(cls), 256, int(bool(sorted_))))
+    def set_class_config(self, cls: int, sorted_: bool = False, block: int = 0):
+        """Opcode grouping and workgroup size (64 / 128 / 256 envs, 0: the
+        default) of one ROM class's launches."""
+        _check(load_library().fa_server_set_class_config(self._h, int(cls), int(block), int(bool(sorted_))))
 
     def set_coalesce_us(self, us: int):
         _check(load_library().fa_server_set_coalesce_us(self._h, int(us)))

@@ -220,6 +220,9 @@ int fa_server_wait(fa_server *s, int client_id, uint8_t *obs_out, double *latenc
 int fa_server_download(fa_server *s, int client_id, AtariStateGpu *states, uint64_t *cpu_cycles,
                        uint64_t *frame_cycles, uint32_t *status);
 int fa_server_client_info(fa_server *s, int client_id, size_t *cls, size_t *n_envs);
+/* Launch shape of one ROM class: workgroup size (64, 128 or 256 envs; 0 picks
+ * the default, the largest whose LDS footprint leaves room on a CU for
+ * another class's workgroup) and opcode grouping. */
 int fa_server_set_class_config(fa_server *s, size_t cls, uint32_t block, int sorted);
 /* Coalescing window (default 300 us): after a launch of a class completes,
  * its next launch waits up to this long for the class's clients to resubmit
