@@ -315,6 +315,79 @@ struct Emu {
         }
     }
 
+    // HeadlessTia::read / write for register numbers that differ between
+    // lanes (the per-lane paths).  tia_read / tia_write above branch on the
+    // register, which is cheap when it is wave-uniform but, on a per-lane
+    // register, runs the whole case chain under exec masks whenever any lane
+    // touches the TIA.  Here the single-byte registers are one computed load
+    // or one predicated store with selects; only the multi-byte ones (RESPx,
+    // GRP0 / GRP1, HMOVE / HMCLR / CXCLR) branch, on a ballot.
+    __device__ __forceinline__ uint32_t tia_read_lanes(uint32_t addr) {
+        const uint32_t r = addr & 0x0Fu;
+        // CXxx at S_CX + r (r < 8), paddles at S_PAD0 + r - 8, INPT4/5 at S_INPT4 + r - 12
+        const uint32_t off = pick(r < 8u, S_CX + r, pick(r < 10u, S_PAD0 - 8u + r, S_INPT4 - 12u + r));
+        const uint32_t x = ld8(img + pick(r < 14u, off, S_CX));
+        const uint32_t pad = pcount >= x ? 0x80u : 0u;
+        const uint32_t inp = x ? 0x80u : 0u;
+        return pick(r < 8u, x, pick(r < 10u, pad, pick(r < 12u, 0x80u, pick(r < 14u, inp, 0u))));
+    }
+    __device__ __forceinline__ void tia_write_lanes(uint32_t addr, uint32_t v) {
+        const uint32_t r = addr & 0x3Fu;
+        // single-byte registers: NUSIZ0..PF2 (4-15), ENAM0/ENAM1/ENABL (0x1D-0x1F),
+        // HMxx (0x20-0x24), VDELxx / RESMPx (0x25-0x29) sit at S_ offsets linear in r
+        const bool g1 = (r - 4u) < 12u, g2 = (r - 0x1Du) < 3u, g3 = (r - 0x20u) < 5u, g4 = (r - 0x25u) < 5u;
+        const bool vbl = r == 0x01u;
+        const uint32_t off = pick(g1, 201u + r, pick(g2, 167u + r, pick(g3, 154u + r, pick(g4, 163u + r, S_VBLANK))));
+        const bool sh3 = (r - 0x0Bu) < 2u;                          // REFP0/1: bit 3
+        const bool sh1 = g2 | ((r - 0x28u) < 2u);                  // ENAMx/ENABL, RESMPx: bit 1
+        const bool bit0 = (r - 0x25u) < 3u;                         // VDELxx: bit 0
+        const uint32_t val = pick(sh3, (v >> 3) & 1u, pick(sh1, (v >> 1) & 1u, pick(bit0, v & 1u, v)));
+        st8((g1 | g2 | g3 | g4 | vbl) ? img + off : junk, val);
+        // VSYNC: a rising bit 1 starts a frame; VBLANK bit 7 dumps the paddle; WSYNC
+        const bool vs = r == 0x00u;
+        const bool rise = vs & !(vsync & 2u) & ((v & 2u) != 0u);
+        line = pick(rise, 0u, line);
+        fcomp = pick(rise, 1u, fcomp);
+        vsync = pick(vs, v, vsync);
+        const bool dump = (v & 0x80u) != 0u;
+        pcount = pick(vbl & dump, 0u, pcount);
+        pdump = pick(vbl, dump ? 1u : 0u, pdump);
+        wsync = pick(r == 0x02u, 1u, wsync);
+        // RESP0..RESBL: position from the clock at the start of the instruction
+        const bool resp = (r - 0x10u) < 5u;
+        if (__ballot(resp) != 0) {
+            const uint32_t pos = resp_pos_of(clock);
+            const uint32_t po = img + S_POS + 2u * (r - 0x10u);
+            st8(resp ? po : junk, pos);
+            st8(resp ? po + 1u : junk, pos >> 8);
+        }
+        const bool gp0 = r == 0x1Bu, gp1 = r == 0x1Cu;
+        if (__ballot(gp0 | gp1) != 0) {
+            // GRP0: grp0_old = grp0; GRP1: grp1_old = grp1 and enabl_old = enabl
+            const uint32_t g = ld8(img + (gp0 ? S_GRP0 : S_GRP1));
+            st8((gp0 | gp1) ? img + (gp0 ? S_GRP0_OLD : S_GRP1_OLD) : junk, g);
+            st8((gp0 | gp1) ? img + (gp0 ? S_GRP0 : S_GRP1) : junk, v);
+            const uint32_t en = ld8(img + S_ENABL);
+            st8(gp1 ? img + S_ENABL_OLD : junk, en);
+        }
+        if (__ballot((r - 0x2Au) < 3u) != 0) {
+            if (r == 0x2Au) {
+                // HMOVE: pos - ((hm as i8) >> 4), rem_euclid 160
+                for (uint32_t i = 0; i < 5u; ++i) {
+                    int32_t o = ((int32_t)(int8_t)ld8(img + S_HM + i)) >> 4;
+                    int32_t q = (int32_t)ld16(img + S_POS + 2u * i) - o;
+                    q += q < 0 ? 160 : 0;
+                    q -= q >= 160 ? 160 : 0;
+                    st16(img + S_POS + 2u * i, (uint32_t)q);
+                }
+            } else if (r == 0x2Bu) {
+                for (uint32_t i = 0; i < 5u; ++i) st8(img + S_HM + i, 0);
+            } else if (r == 0x2Cu) {
+                for (uint32_t i = 0; i < 8u; ++i) st8(img + S_CX + i, 0);
+            }
+        }
+    }
+
     // ------------------------------------------------------------ bus
     __device__ __forceinline__ uint32_t ram_off(uint32_t a) const { return img + S_RAM + (a & 0x7Fu); }
 
@@ -327,12 +400,12 @@ struct Emu {
     // TIA / RIOT I/O and the flat-array padding (src/atari.rs:509-519)
     __device__ __forceinline__ uint32_t io_read(uint32_t a) {
         if ((a & 0x280u) == 0x280u) return pia_read(a);
-        if (!(a & 0x80u)) return tia_read(a);
+        if (!(a & 0x80u)) return tia_read_lanes(a);
         return 0u;
     }
     __device__ __forceinline__ void io_write(uint32_t a, uint32_t v) {
         if (a & 0x200u) pia_write(a, v);
-        else tia_write(a, v);
+        else tia_write_lanes(a, v);
     }
     // HeadlessBus::read (src/atari.rs:497-520): ROM window and the two RAM
     // mirrors come from LDS, everything else from the TIA / RIOT.
@@ -364,12 +437,12 @@ struct Emu {
     // Zero page and stack page: bit 7 selects RAM, otherwise the TIA.
     __device__ __forceinline__ uint32_t zp_read(uint32_t a8) {
         uint32_t v = ld8(ram_off(a8));
-        if (!(a8 & 0x80u)) v = tia_read(a8);
+        if (!(a8 & 0x80u)) v = tia_read_lanes(a8);
         return v;
     }
     __device__ __forceinline__ void zp_write(uint32_t a8, uint32_t v) {
         st8((a8 & 0x80u) ? ram_off(a8) : junk, v);
-        if (!(a8 & 0x80u)) tia_write(a8, v);
+        if (!(a8 & 0x80u)) tia_write_lanes(a8, v);
     }
     __device__ __forceinline__ void push(uint32_t v) { zp_write(sp, v); sp = (sp - 1u) & 0xFFu; }
     __device__ __forceinline__ uint32_t pull() { sp = (sp + 1u) & 0xFFu; return zp_read(sp); }

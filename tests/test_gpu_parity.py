@@ -101,6 +101,30 @@ def test_config3_branchy(fa, orc, W, sorted_):
         assert_same(gpu, ref, f"frame {f}")
 
 
+@pytest.mark.parametrize("case", ["identical", "few_differ"])
+def test_branchy_uniform_waves(fa, orc, W, case):
+    """The branchy ROM -- every documented opcode: JSR/RTS, BRK/RTI, JMP (ind),
+    stack pairs, indexed and indirect modes, decimal ADC/SBC, F6 bank switches
+    through hotspot reads -- with every env given the same actions: whole waves
+    follow one control path, so each instruction runs in the wave-uniform mode
+    with its scalar control state.  In "few_differ" one env in every 64 has its
+    own actions, so its wave leaves that mode at the first branch on the
+    differing data and runs on the per-lane paths from there."""
+    rom = W.rom_for(3)
+    n, frames = 512, 10
+    gpu, ref = _pair(fa, orc, rom, n)
+    base = W.ActionStream(W.config_seed(3), 1)
+    other = W.ActionStream(W.config_seed(3) + 7, n)
+    for f in range(frames):
+        a = np.full(n, base.next()[0], np.uint8)
+        o = other.next()
+        if case == "few_differ":
+            a[::64] = o[::64]
+        gpu.step(a)
+        ref.run(a[None, :])
+        assert_same(gpu, ref, f"{case} frame {f}")
+
+
 @pytest.mark.parametrize("member", [0, 1, 2, 3])
 def test_mix_frameskip(fa, orc, W, member):
     """Each ROM of the heterogeneous mix with per-env frameskip 1-4."""
