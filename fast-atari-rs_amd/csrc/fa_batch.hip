@@ -216,11 +216,11 @@ int fa_batch_create(const uint8_t *const *roms, const size_t *rom_lens, const si
         hipEventCreate(&b->t0) != hipSuccess || hipEventCreate(&b->t1) != hipSuccess)
         return cleanup(fail(FA_EHIP, "stream / event creation failed"));
     const size_t n1 = std::max<size_t>(b->n, 1), s1 = std::max<size_t>(b->slots, 1);
-    if (hipMalloc(&b->d_desc, 2048) != hipSuccess || hipMalloc(&b->d_states, s1 * fa::kStateBytes) != hipSuccess ||
+    if (hipMalloc(&b->d_desc, fa::kDescBytes) != hipSuccess || hipMalloc(&b->d_states, s1 * fa::kStateBytes) != hipSuccess ||
         hipMalloc(&b->d_cpu, n1 * 8) != hipSuccess || hipMalloc(&b->d_fcyc, n1 * 8) != hipSuccess ||
         hipMalloc(&b->d_instr, n1 * 8) != hipSuccess || hipMalloc(&b->d_status, n1 * 4) != hipSuccess)
         return cleanup(fail(FA_ENOMEM, "device allocation failed"));
-    uint32_t desc[512];
+    uint32_t desc[fa::kDescBytes / 4];
     fa::descriptor_table(desc);
     if (hipMemcpy(b->d_desc, desc, sizeof(desc), hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(FA_EHIP, "descriptor upload failed"));

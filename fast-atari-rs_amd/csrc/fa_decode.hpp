@@ -54,6 +54,85 @@ enum : uint32_t { R_A = 0, R_X = 1, R_Y = 2, R_SP = 3, R_NONE = 4 };
 enum : uint32_t { SP_JMP = 0, SP_JMPI = 1, SP_JSR = 2, SP_RTS = 3, SP_RTI = 4, SP_BRK = 5,
                   SP_PHA = 6, SP_PHP = 7, SP_PLA = 8, SP_PLP = 9 };
 
+// ---- d2, d3: the same decode as one-hot bits and ready-made flag masks, for
+// the per-lane path, where every selection is an AND with a sign-extended bit
+// (one v_bfe_i32) instead of a compare on a field.  Derived from d0 / d1.
+enum : uint32_t {
+    G_SRC = 0,              // 4 bits: source register A, X, Y, SP (one-hot)
+    G_DST = 4,              // 4 bits: destination register A, X, Y, SP (one-hot; none: 0)
+    G_R_LOG = 8,            // result: logic (AND / EOR / ORA / loads / BIT's A & v)
+    G_R_ARI = 9,            // result: binary add with carry
+    G_R_SHL = 10,           // result: ASL / ROL
+    G_R_SHR = 11,           // result: LSR / ROR
+    G_R_INC = 12,           // result: operand + delta (INC, DEC, INX .. DEY, transfers)
+    G_R_SRC = 13,           // result: the source register (stores)
+    G_CIN_SHL = 14,         // ROL: carry into bit 0
+    G_CIN_SHR = 15,         // ROR: carry into bit 7
+    G_DELTA = 16,           // 8 bits: delta of G_R_INC (0, 1 or $FF)
+    G_IX_X = 24,            // X added to the base address
+    G_IX_Y = 25,            // Y added to the base (absolute / zero-page Y, (zp),Y)
+    G_IZX = 26,             // (zp,X): X added to the zero-page pointer address
+    G_IND = 27,             // base address read from a zero-page pointer
+    G_BIT = 28,             // BIT: N and V from the operand
+    G_C_SUM = 29,           // new carry: bit 8 of the sum
+    G_C_B7 = 30,            // new carry: operand bit 7 (ASL / ROL)
+    G_C_B0 = 31,            // new carry: operand bit 0 (LSR / ROR)
+};
+enum : uint32_t {
+    H_CLR = 0,              // 8 bits: status bits this instruction writes
+    H_TAKE = 8,             // 8 bits: status bits taken from the computed N, V, Z, C
+    H_SET = 16,             // 8 bits: status bits forced to 1 (SEC, SED, SEI)
+    H_M_AND = 24, H_M_XOR = 25, H_M_LD = 26,   // logic masks (d0's D_M_*, BIT: AND)
+};
+constexpr uint32_t kDescWords = 4;                  // words per opcode
+constexpr uint32_t kDescBytes = 256u * kDescWords * 4u;
+
+// d2 and d3 of an opcode from its d0 and d1.
+constexpr void describe_lanes(uint32_t d0, uint32_t d1, uint32_t &d2, uint32_t &d3) {
+    d2 = 0;
+    d3 = 0;
+    if ((d0 & 3u) != CLS_COMPUTE) return;
+    const uint32_t kind = (d0 >> D_KIND_SHIFT) & 7u;
+    const uint32_t src = (d1 >> D1_SRC_SHIFT) & 7u, dst = (d1 >> D1_DST_SHIFT) & 7u;
+    const uint32_t rop = (d1 >> D1_RMW_SHIFT) & 7u, dl = (d1 >> D1_DELTA_SHIFT) & 3u;
+    const uint32_t ind = (d0 >> D_IND_SHIFT) & 3u, idx = (d0 >> D_IDX_SHIFT) & 3u;
+    constexpr uint32_t N = 0x80, V = 0x40, Z = 0x02, C = 0x01;
+    d2 |= 1u << (G_SRC + (src & 3u));
+    if (dst < 4u) d2 |= 1u << (G_DST + dst);
+    uint32_t delta = 0;
+    switch (kind) {
+    case K_LOGIC: d2 |= 1u << G_R_LOG; break;
+    case K_ARITH: d2 |= 1u << G_R_ARI; break;
+    case K_BIT: d2 |= (1u << G_R_LOG) | (1u << G_BIT); d3 |= 1u << H_M_AND; break;
+    case K_RMW:
+        if (rop < 2u) d2 |= (1u << G_R_SHL) | (rop == 1u ? 1u << G_CIN_SHL : 0u);
+        else if (rop < 4u) d2 |= (1u << G_R_SHR) | (rop == 3u ? 1u << G_CIN_SHR : 0u);
+        else { d2 |= 1u << G_R_INC; delta = rop == 5u ? 1u : 0xFFu; }
+        break;
+    case K_MOVE: d2 |= 1u << G_R_INC; delta = dl == 1u ? 1u : (dl == 2u ? 0xFFu : 0u); break;
+    default: d2 |= 1u << G_R_SRC; break;
+    }
+    d2 |= delta << G_DELTA;
+    if (ind == 1u) d2 |= (1u << G_IND) | (1u << G_IZX);
+    else if (ind == 2u) d2 |= (1u << G_IND) | (1u << G_IX_Y);
+    else if (idx == 1u) d2 |= 1u << G_IX_X;
+    else if (idx == 2u) d2 |= 1u << G_IX_Y;
+    uint32_t take = 0;
+    if (d0 & D_SETNZ) take |= N | Z;
+    if (d0 & D_SETV) take |= V;
+    if (kind == K_BIT) take |= N | V | Z;
+    if (d0 & D_SETC) {
+        take |= C;
+        d2 |= kind == K_ARITH ? 1u << G_C_SUM : (rop < 2u ? 1u << G_C_B7 : 1u << G_C_B0);
+    }
+    const uint32_t fm = kind == K_FLAGS ? (d1 >> D1_FMASK_SHIFT) & 0xFFu : 0u;
+    const uint32_t fv = kind == K_FLAGS ? (d1 >> D1_FVAL_SHIFT) & 0xFFu : 0u;
+    d3 |= ((take | fm) << H_CLR) | (take << H_TAKE) | (fv << H_SET);
+    if (d0 & D_M_AND) d3 |= 1u << H_M_AND;
+    if (d0 & D_M_XOR) d3 |= 1u << H_M_XOR;
+    if (d0 & D_M_LD) d3 |= 1u << H_M_LD;
+}
+
 namespace detail {
 
 enum Mode : uint8_t { IMP, ACC, IMM, ZP, ZPX, ZPY, ABS, ABX, ABY, IZX, IZY, REL, IND };
@@ -236,18 +315,21 @@ constexpr uint64_t descriptor_of(uint32_t code) {
     X(0xE0) X(0xE1) X(0xE4) X(0xE5) X(0xE6) X(0xE8) X(0xE9) X(0xEA) X(0xEC) X(0xED) X(0xEE) X(0xF0)       \
     X(0xF1) X(0xF5) X(0xF6) X(0xF8) X(0xF9) X(0xFD) X(0xFE)
 
-// Fills table[2 * op] = d0, table[2 * op + 1] = d1 for all 256 opcodes;
-// undocumented opcodes are CLS_ILLEGAL with length 1.
+// Fills table[4 * op + k] = dk (k = 0..3) for all 256 opcodes; undocumented
+// opcodes are CLS_ILLEGAL with length 1.
 inline void build_descriptors(uint32_t *table) {
     for (int i = 0; i < 256; ++i) {
-        table[2 * i] = (CLS_ILLEGAL << D_CLS_SHIFT) | (1u << D_LEN_SHIFT);
-        table[2 * i + 1] = 0;
+        table[4 * i] = (CLS_ILLEGAL << D_CLS_SHIFT) | (1u << D_LEN_SHIFT);
+        table[4 * i + 1] = table[4 * i + 2] = table[4 * i + 3] = 0;
     }
     for (const detail::Op &o : detail::kOps) {
-        uint32_t d0 = 0, d1 = 0;
+        uint32_t d0 = 0, d1 = 0, d2 = 0, d3 = 0;
         detail::describe(o, d0, d1);
-        table[2 * o.code] = d0;
-        table[2 * o.code + 1] = d1;
+        describe_lanes(d0, d1, d2, d3);
+        table[4 * o.code] = d0;
+        table[4 * o.code + 1] = d1;
+        table[4 * o.code + 2] = d2;
+        table[4 * o.code + 3] = d3;
     }
 }
 

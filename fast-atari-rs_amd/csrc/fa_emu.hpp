@@ -16,8 +16,8 @@
 // accesses, decimal-mode arithmetic and stack / jump instructions branch.
 //
 // LDS layout of one workgroup (byte offsets into fa_lds):
-//   [0, 2048)                     opcode descriptors (256 x 8 bytes)
-//   [2048, 2048 + rom_bytes)      ROM image: the 4 KB window of a Fixed ROM,
+//   [0, 4096)                     opcode descriptors (256 x 16 bytes)
+//   [4096, 4096 + rom_bytes)      ROM image: the 4 KB window of a Fixed ROM,
 //                                 or every 4 KB bank of an F8/F6/F4 ROM
 //   [.., + bs * 222)              the workgroup's AtariStateGpu images (AoS,
 //                                 222-byte stride), staged with 16-byte loads
@@ -35,7 +35,7 @@
 namespace fa {
 
 constexpr uint32_t DESC0 = 0;        // LDS offset of the descriptor table
-constexpr uint32_t ROM0 = 2048;      // LDS offset of the ROM image
+constexpr uint32_t ROM0 = kDescBytes; // LDS offset of the ROM image
 
 // Byte offsets inside the 222-byte AtariStateGpu (src/cuda_env.rs:12-38).
 enum : uint32_t {
@@ -70,6 +70,9 @@ __device__ __forceinline__ void st8(uint32_t off, uint32_t v) { fa_lds[off] = (u
 __device__ __forceinline__ uint32_t ld32(uint32_t off) {
     return *reinterpret_cast<const uint32_t *>(fa_lds + off);
 }
+__device__ __forceinline__ uint4 ld128(uint32_t off) {
+    return *reinterpret_cast<const uint4 *>(fa_lds + off);
+}
 __device__ __forceinline__ uint32_t ld16(uint32_t off) {
     return (uint32_t)fa_lds[off] | ((uint32_t)fa_lds[off + 1] << 8);
 }
@@ -84,6 +87,8 @@ __device__ __forceinline__ void st16(uint32_t off, uint32_t v) {
 // into branches).
 __device__ __forceinline__ uint32_t mk(bool c) { return 0u - (uint32_t)c; }
 __device__ __forceinline__ uint32_t pick(bool c, uint32_t t, uint32_t f) { return (t & mk(c)) | (f & ~mk(c)); }
+// All-ones when bit k of w is set (one v_bfe_i32).
+__device__ __forceinline__ uint32_t bm(uint32_t w, uint32_t k) { return (uint32_t)((int32_t)(w << (31u - k)) >> 31); }
 
 // ---------------------------------------------------------------- RIOT timer
 // Pia::tick_n (src/pia.rs:107-166) with the prescaler division done by shifts
@@ -452,18 +457,17 @@ struct Emu {
     // and below the hotspot / vector area the fetches have no side effects
     // and come straight from LDS; elsewhere they go through the bus in the
     // reference's order (opcode, then operands or the phantom read of PC).
-    __device__ __forceinline__ void fetch(uint32_t &b1, uint32_t &b2, uint32_t &d0, uint32_t &d1, uint32_t &op) {
+    __device__ __forceinline__ void fetch(uint32_t &b1, uint32_t &b2, uint4 &d, uint32_t &op) {
         if ((pc & 0x1000u) && (pc & 0xFFFu) < 0xFF0u) {
             const uint32_t o = ROM0 + (bank << 12) + (pc & 0xFFFu);
             op = ld8(o);
             b1 = ld8(o + 1u);
             b2 = ld8(o + 2u);
-            d0 = ld32(DESC0 + op * 8u);
-            d1 = ld32(DESC0 + op * 8u + 4u);
+            d = ld128(DESC0 + op * 16u);
         } else {
             op = bus_read(pc);
-            d0 = ld32(DESC0 + op * 8u);
-            d1 = ld32(DESC0 + op * 8u + 4u);
+            d = ld128(DESC0 + op * 16u);
+            const uint32_t d0 = d.x;
             const uint32_t len = (d0 >> D_LEN_SHIFT) & 3u;
             const bool illegal = (d0 & 3u) == CLS_ILLEGAL;
             b1 = len >= 2u ? bus_read((pc + 1u) & 0xFFFFu) : 0u;
@@ -476,106 +480,133 @@ struct Emu {
     __device__ __forceinline__ void set_zn(uint32_t v) {
         p = (p & ~(FZ | FN)) | (v & 0x80u) | (v == 0u ? FZ : 0u);
     }
-    __device__ __forceinline__ uint32_t reg(uint32_t sel) const {
-        return (a & mk(sel == R_A)) | (x & mk(sel == R_X)) | (y & mk(sel == R_Y)) | (sp & mk(sel == R_SP));
-    }
-    __device__ __forceinline__ void set_reg(uint32_t sel, uint32_t v) {
-        a = sel == R_A ? v : a;
-        x = sel == R_X ? v : x;
-        y = sel == R_Y ? v : y;
-        sp = sel == R_SP ? v : sp;
+
+    // Decimal-mode ADC / SBC (src/cpu.rs:155-166, 180-191) for the lanes in
+    // `dec`, given the binary-mode operand `v`, the pre-instruction A and P and
+    // the status already updated by the binary path (ADC keeps its Z; SBC keeps
+    // every binary flag, only the accumulator differs).  Both forms are computed
+    // and selected per lane.
+    __device__ __forceinline__ void decimal_lanes(bool dec, bool sub, uint32_t a0, uint32_t v, uint32_t p0) {
+        const uint32_t c = p0 & FC;
+        uint32_t lo = (a0 & 0xFu) + (v & 0xFu) + c;
+        lo += lo > 9u ? 6u : 0u;
+        uint32_t hi = (a0 >> 4) + (v >> 4) + (lo > 0xFu ? 1u : 0u);
+        const uint32_t inter = ((hi << 4) | (lo & 0xFu)) & 0xFFu;
+        uint32_t np = p & ~(FN | FV | FC);
+        np |= (hi & 8u) ? FN : 0u;
+        np |= (~(a0 ^ v) & (a0 ^ inter) & 0x80u) ? FV : 0u;
+        hi += hi > 9u ? 6u : 0u;
+        np |= hi > 0xFu ? FC : 0u;
+        const uint32_t a_add = ((hi & 0xFu) << 4) | (lo & 0xFu);
+        int32_t slo = (int32_t)(a0 & 0xFu) - (int32_t)(v & 0xFu) - (int32_t)(c ^ 1u);
+        const uint32_t lb = slo < 0 ? 1u : 0u;
+        slo -= lb ? 6 : 0;
+        int32_t shi = (int32_t)(a0 >> 4) - (int32_t)(v >> 4) - (int32_t)lb;
+        shi -= shi < 0 ? 6 : 0;
+        const uint32_t a_sub = (((uint32_t)shi & 0xFu) << 4) | ((uint32_t)slo & 0xFu);
+        a = pick(dec, pick(sub, a_sub, a_add), a);
+        p = pick(dec & !sub, np, p);
     }
 
-    // Decimal-mode ADC / SBC accumulator and flags (src/cpu.rs:155-166, 180-191),
-    // given the binary-mode operand `v` and the pre-instruction A and P.
-    __device__ __forceinline__ void decimal(bool sub, uint32_t a0, uint32_t v, uint32_t p0) {
-        const uint32_t c = p0 & FC;
-        if (!sub) {
-            uint32_t lo = (a0 & 0xFu) + (v & 0xFu) + c;
-            if (lo > 9u) lo += 6u;
-            uint32_t hi = (a0 >> 4) + (v >> 4) + (lo > 0xFu ? 1u : 0u);
-            const uint32_t inter = ((hi << 4) | (lo & 0xFu)) & 0xFFu;
-            uint32_t np = p & ~(FN | FV | FC);           // Z stays the binary one
-            np |= (hi & 8u) ? FN : 0u;
-            np |= (~(a0 ^ v) & (a0 ^ inter) & 0x80u) ? FV : 0u;
-            if (hi > 9u) hi += 6u;
-            np |= hi > 0xFu ? FC : 0u;
-            p = np;
-            a = ((hi & 0xFu) << 4) | (lo & 0xFu);
-        } else {
-            // all flags are the binary ones; only the accumulator differs
-            const uint32_t borrow = c ^ 1u;
-            int32_t lo = (int32_t)(a0 & 0xFu) - (int32_t)(v & 0xFu) - (int32_t)borrow;
-            const uint32_t lb = lo < 0 ? 1u : 0u;
-            lo -= lb ? 6 : 0;
-            int32_t hi = (int32_t)(a0 >> 4) - (int32_t)(v >> 4) - (int32_t)lb;
-            hi -= hi < 0 ? 6 : 0;
-            a = (((uint32_t)hi & 0xFu) << 4) | ((uint32_t)lo & 0xFu);
+    // Compute-class instruction (loads, stores, ALU, read-modify-write,
+    // transfers, flag instructions) on the per-lane path: every lane runs the
+    // same straight-line code, steered by the one-hot words d2 / d3, and only
+    // TIA / RIOT accesses, zero-page pointers in the TIA and decimal mode
+    // branch, each on a ballot of the lanes that need it.  Bus accesses are
+    // those of src/cpu.rs in the reference's order; the phantom write of a
+    // read-modify-write is made only where it can be seen (TIA / RIOT), since
+    // on RAM the final write replaces it and on ROM both writes select the
+    // same bank.
+    __device__ __forceinline__ uint32_t compute_lanes(uint32_t b1, uint32_t b2, uint32_t d0, uint32_t g, uint32_t h) {
+        uint32_t cyc = (d0 >> D_CYC_SHIFT) & 7u;
+        const uint32_t s = (a & bm(g, G_SRC)) | (x & bm(g, G_SRC + 1u)) | (y & bm(g, G_SRC + 2u)) |
+                           (sp & bm(g, G_SRC + 3u));
+        // effective address (every lane computes one; only D_MEM lanes use it)
+        const uint32_t zpm = bm(d0, 8u);                         // D_ZP: 8-bit address arithmetic
+        uint32_t base = b1 | ((b2 << 8) & ~zpm);
+        if (g & (1u << G_IND)) {
+            // pointer from the zero page; pointer bytes in the TIA read its registers
+            const uint32_t zp = (b1 + (x & bm(g, G_IZX))) & 0xFFu;
+            const uint32_t zq = (zp + 1u) & 0xFFu;
+            uint32_t lo = ld8(ram_off(zp)), hi = ld8(ram_off(zq));
+            if (__ballot(((zp & zq) & 0x80u) == 0u) != 0) {
+                lo = (zp & 0x80u) ? lo : tia_read_lanes(zp);
+                hi = (zq & 0x80u) ? hi : tia_read_lanes(zq);
+            }
+            base = lo | (hi << 8);
         }
+        const uint32_t ea = (base + ((x & bm(g, G_IX_X)) | (y & bm(g, G_IX_Y)))) & (0xFFFFu ^ (0xFF00u & zpm));
+        const bool cross = ((base ^ ea) & 0xFF00u) != 0u;
+        const bool cross_fix = ((d0 & D_DUMMY_CROSS) != 0u) & cross;
+        touch_if(((d0 & D_DUMMY_ALWAYS) != 0u) | cross_fix, (base & 0xFF00u) | (ea & 0xFFu));
+        cyc += cross_fix ? 1u : 0u;
+        // operand: immediate, register or the data read
+        uint32_t v = (d0 & D_IMM) ? b1 : s;
+        {
+            const bool rd = (d0 & D_RD) != 0u;
+            const uint32_t a13 = ea & 0x1FFFu;
+            const bool rom = (a13 & 0x1000u) != 0u;
+            hotspot_if(rd, a13);
+            const bool ram = (a13 & 0x1E80u) == 0x0080u;
+            const uint32_t m = ld8(rd ? (rom ? ROM0 + (bank << 12) + (a13 & 0xFFFu) : ram_off(a13)) : junk);
+            v = rd ? m : v;
+            const bool io = rd & !rom & !ram;
+            if (__ballot(io) != 0) {
+                if (io) v = io_read(a13);
+            }
+        }
+        // result candidates and the selected one
+        const uint32_t p0 = p, a0 = a;
+        const uint32_t lg = (a & v & bm(h, H_M_AND)) | ((a ^ v) & bm(h, H_M_XOR)) | (v & bm(h, H_M_LD));
+        const uint32_t vv = v ^ (0xFFu & bm(d0, 25u));           // D_INV
+        const uint32_t cin = p & FC;
+        const uint32_t sum = s + vv + (cin | ((d0 >> 26) & 1u));  // D_CIN1
+        const uint32_t ar = sum & 0xFFu;
+        const uint32_t shl = ((v << 1) | (cin & bm(g, G_CIN_SHL))) & 0xFFu;
+        const uint32_t shr = (v >> 1) | ((cin << 7) & bm(g, G_CIN_SHR));
+        const uint32_t inc = (v + (g >> G_DELTA)) & 0xFFu;       // bits above the delta fall off with the mask
+        const uint32_t r = (lg & bm(g, G_R_LOG)) | (ar & bm(g, G_R_ARI)) | (shl & bm(g, G_R_SHL)) |
+                           (shr & bm(g, G_R_SHR)) | (inc & bm(g, G_R_INC)) | (s & bm(g, G_R_SRC));
+        // status: N, V, Z, C candidates, then the descriptor's clear / take / set masks
+        const uint32_t mbit = bm(g, G_BIT);
+        const uint32_t nv = (v & mbit) | (r & ~mbit);
+        const uint32_t vf = (((~(s ^ vv) & (s ^ ar)) >> 1) & FV & ~mbit) | (v & FV & mbit);
+        const uint32_t cf = (((sum >> 8) & bm(g, G_C_SUM)) | ((v >> 7) & bm(g, G_C_B7)) | (v & bm(g, G_C_B0))) & FC;
+        const uint32_t cand = (nv & FN) | vf | (r == 0u ? FZ : 0u) | cf;
+        p = (p & ~(h & 0xFFu)) | (cand & (h >> H_TAKE)) | ((h >> H_SET) & 0xFFu);
+        const uint32_t ma = bm(g, G_DST), mx = bm(g, G_DST + 1u), my = bm(g, G_DST + 2u), ms = bm(g, G_DST + 3u);
+        a = (r & ma) | (a & ~ma);
+        x = (r & mx) | (x & ~mx);
+        y = (r & my) | (y & ~my);
+        sp = (r & ms) | (sp & ~ms);
+        const bool dec = ((d0 & D_DEC) != 0u) & ((p0 & FD) != 0u);
+        if (__ballot(dec) != 0) decimal_lanes(dec, (d0 & D_INV) != 0u, a0, v, p0);
+        // write-back
+        {
+            const bool wr = (d0 & D_WR) != 0u;
+            const uint32_t a13 = ea & 0x1FFFu;
+            hotspot_if(wr, a13);
+            const bool rom = (a13 & 0x1000u) != 0u;
+            const bool ram = (a13 & 0x1280u) == 0x0080u;
+            st8(wr & ram ? ram_off(a13) : junk, r);
+            const bool io = wr & !rom & !ram;
+            if (__ballot(io) != 0) {
+                if (io) {
+                    if (d0 & D_RMWMEM) io_write(a13, v);
+                    io_write(a13, r);
+                }
+            }
+        }
+        return cyc;
     }
 
     // Executes one decoded instruction (pc already points past it).  Returns
     // the cycle count, or 0 for an illegal opcode (src/cpu.rs:1139-1141).
-    __device__ __forceinline__ uint32_t execute(uint32_t b1, uint32_t b2, uint32_t d0, uint32_t d1) {
+    __device__ __forceinline__ uint32_t execute(uint32_t b1, uint32_t b2, const uint4 &d) {
+        const uint32_t d0 = d.x, d1 = d.y;
         uint32_t cyc = (d0 >> D_CYC_SHIFT) & 7u;
         const uint32_t cls = d0 & 3u;
-        if (cls == CLS_COMPUTE) {
-            const uint32_t kind = (d0 >> D_KIND_SHIFT) & 7u;
-            const uint32_t s = reg((d1 >> D1_SRC_SHIFT) & 7u);
-            uint32_t v = (d0 & D_IMM) ? b1 : s;
-            uint32_t ea = 0;
-            if (d0 & D_MEM) {
-                const uint32_t ind = (d0 >> D_IND_SHIFT) & 3u;
-                uint32_t ixsel = (d0 >> D_IDX_SHIFT) & 3u;
-                uint32_t base = (d0 & D_ZP) ? b1 : (b1 | (b2 << 8));
-                if (ind) {
-                    // pointer fetch from the zero page (no side effects)
-                    const uint32_t zp = ind == 1u ? ((b1 + x) & 0xFFu) : b1;
-                    base = zp_read(zp) | (zp_read((zp + 1u) & 0xFFu) << 8);
-                    ixsel = ind == 2u ? 2u : 0u;
-                }
-                const uint32_t idx = ixsel == 1u ? x : (ixsel == 2u ? y : 0u);
-                ea = (base + idx) & ((d0 & D_ZP) ? 0xFFu : 0xFFFFu);
-                const bool cross = ((base ^ ea) & 0xFF00u) != 0u;
-                const bool fix = (d0 & D_DUMMY_ALWAYS) || ((d0 & D_DUMMY_CROSS) && cross);
-                touch_if(fix, (base & 0xFF00u) | (ea & 0xFFu));
-                cyc += ((d0 & D_DUMMY_CROSS) && cross) ? 1u : 0u;
-                if (d0 & D_RD) v = bus_read(ea);
-            }
-            const uint32_t p0 = p, a0 = a;
-            // candidates
-            const uint32_t mand = (d0 & D_M_AND) ? 0xFFu : 0u, mxor = (d0 & D_M_XOR) ? 0xFFu : 0u,
-                           mld = (d0 & D_M_LD) ? 0xFFu : 0u;
-            const uint32_t lg = ((a & v) & mand) | ((a ^ v) & mxor) | (v & mld);
-            const uint32_t vv = (d0 & D_INV) ? (v ^ 0xFFu) : v;
-            const uint32_t sum = s + vv + ((d0 & D_CIN1) ? 1u : (p & FC));
-            const uint32_t ar = sum & 0xFFu;
-            const uint32_t rop = (d1 >> D1_RMW_SHIFT) & 7u;
-            const uint32_t cin = p & FC;
-            const uint32_t shl = ((v << 1) | (rop == 1u ? cin : 0u)) & 0xFFu;
-            const uint32_t shr = (v >> 1) | (rop == 3u ? (cin << 7) : 0u);
-            const uint32_t idc = (v + (rop == 5u ? 1u : 0xFFu)) & 0xFFu;
-            const uint32_t rr = rop < 2u ? shl : (rop < 4u ? shr : idc);
-            const uint32_t dl = (d1 >> D1_DELTA_SHIFT) & 3u;
-            const uint32_t mv = (s + (dl == 1u ? 1u : (dl == 2u ? 0xFFu : 0u))) & 0xFFu;
-            uint32_t r = kind == K_LOGIC ? lg : kind == K_ARITH ? ar : kind == K_RMW ? rr : kind == K_MOVE ? mv : s;
-            // flags
-            const uint32_t c_new = kind == K_ARITH ? (sum >> 8) & 1u : (rop < 2u ? (v >> 7) : (v & 1u));
-            const uint32_t v_new = (~(s ^ vv) & (s ^ ar) & 0x80u) ? FV : 0u;
-            uint32_t np = p;
-            np = (d0 & D_SETNZ) ? ((np & ~(FN | FZ)) | (r & 0x80u) | (r == 0u ? FZ : 0u)) : np;
-            np = (d0 & D_SETC) ? ((np & ~FC) | c_new) : np;
-            np = (d0 & D_SETV) ? ((np & ~FV) | v_new) : np;
-            np = kind == K_BIT ? ((np & ~(FN | FV | FZ)) | (v & 0xC0u) | ((a & v) == 0u ? FZ : 0u)) : np;
-            const uint32_t fm = (d1 >> D1_FMASK_SHIFT) & 0xFFu, fv = d1 >> D1_FVAL_SHIFT;
-            np = kind == K_FLAGS ? ((np & ~fm) | fv) : np;
-            p = np;
-            set_reg((d1 >> D1_DST_SHIFT) & 7u, r);
-            if ((d0 & D_DEC) && (p0 & FD)) decimal((d0 & D_INV) != 0u, a0, v, p0);
-            if (d0 & D_RMWMEM) bus_write(ea, v);
-            if (d0 & D_WR) bus_write(ea, r);
-            return cyc;
-        }
+        if (cls == CLS_COMPUTE) return compute_lanes(b1, b2, d0, d.z, d.w);
         if (cls == CLS_BRANCH) {
             const uint32_t fm = (d1 >> D1_FMASK_SHIFT) & 0xFFu;
             const bool taken = ((p & fm) != 0u) == ((d1 >> D1_FVAL_SHIFT) != 0u);
@@ -803,7 +834,7 @@ struct Emu {
         else if (dst == R_X) x = r;
         else if (dst == R_Y) y = r;
         else if (dst == R_SP) sp = r;
-        if ((u0 & D_DEC) && (p0 & FD)) decimal((u0 & D_INV) != 0u, a0, v, p0);
+        if ((u0 & D_DEC) && (p0 & FD)) decimal_lanes(true, (u0 & D_INV) != 0u, a0, v, p0);
         if (u0 & D_WR) {
             st8(ramw ? ram_off(a13) : junk, r);                 // ROM / I/O targets hit the discard byte
             if (io) {
@@ -863,10 +894,10 @@ struct Emu {
         } while (GROUP && rem);
         if (!done) {
             uint32_t gb1 = b1, gb2 = b2, gop = op;
-            uint32_t gd0 = ld32(DESC0 + op * 8u), gd1 = ld32(DESC0 + op * 8u + 4u);
-            if (!fz) fetch(gb1, gb2, gd0, gd1, gop);
-            pc = (pc0 + ((gd0 >> D_LEN_SHIFT) & 3u)) & 0xFFFFu;
-            cyc = execute(gb1, gb2, gd0, gd1);
+            uint4 gd = ld128(DESC0 + op * 16u);
+            if (!fz) fetch(gb1, gb2, gd, gop);
+            pc = (pc0 + ((gd.x >> D_LEN_SHIFT) & 3u)) & 0xFFFFu;
+            cyc = execute(gb1, gb2, gd);
         }
         return cyc;
     }

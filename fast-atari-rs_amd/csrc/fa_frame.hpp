@@ -119,7 +119,7 @@ __global__ void __launch_bounds__(256) fa_frame_kernel(StepArgs g) {
     {
         const uint4 *dsrc = reinterpret_cast<const uint4 *>(g.desc);
         uint4 *ddst = reinterpret_cast<uint4 *>(fa_lds + DESC0);
-        for (uint32_t i = tid; i < 2048u / 16u; i += bs) ddst[i] = dsrc[i];
+        for (uint32_t i = tid; i < kDescBytes / 16u; i += bs) ddst[i] = dsrc[i];
         const uint4 *src = reinterpret_cast<const uint4 *>(g.rom);
         uint4 *dst = reinterpret_cast<uint4 *>(fa_lds + ROM0);
         for (uint32_t i = tid; i < g.rom_bytes / 16u; i += bs) dst[i] = src[i];

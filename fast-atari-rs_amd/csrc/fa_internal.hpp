@@ -12,11 +12,12 @@
 #include <vector>
 
 #include "../../include/fastatari.h"
+#include "fa_decode.hpp"
 
 namespace fa {
 
 constexpr uint32_t kStateBytes = 222;     // AtariStateGpu
-constexpr uint32_t kLdsRom0 = 2048;       // LDS offset of the ROM image (after the descriptor table)
+constexpr uint32_t kLdsRom0 = kDescBytes; // LDS offset of the ROM image (after the descriptor table)
 constexpr uint32_t kMaxBlock = 256;
 
 // One workgroup's share of a served launch: a client's slot range and its
@@ -48,7 +49,7 @@ struct StepArgs {
     const uint32_t *tasks;        // NULL: workgroup b covers slots [b * bs, b * bs + bs) of [0, n);
                                   // else (first, count) per workgroup, first % 8 == 0, count <= bs
     const HostTask *htasks;       // one per workgroup; replaces tasks, packed and obs when set
-    const uint32_t *desc;         // 512 words: opcode descriptors
+    const uint32_t *desc;         // kDescBytes: opcode descriptors
     const uint8_t *rom;           // ROM image, rom_bytes (multiple of 4096)
     uint32_t rom_bytes;
     uint32_t hot_lo, hot_n;       // hotspot window (lo12 - hot_lo < hot_n)
@@ -80,7 +81,7 @@ hipError_t prepare_sorted_kernel(size_t max_lds);
 hipError_t launch_sorted(const StepArgs &g, uint32_t bs, uint32_t grid, size_t lds, hipStream_t s);
 hipError_t launch_fill(const AtariStateGpu &tmpl, uint8_t *states, uint64_t *cpu, uint64_t *fcyc, uint64_t *instr,
                        uint32_t *status, uint32_t n, uint32_t n_pad, hipStream_t s);
-void descriptor_table(uint32_t out[512]);
+void descriptor_table(uint32_t out[kDescBytes / 4]);
 
 // Errors: the message of the calling thread's last failure.
 int fail(int code, const std::string &msg);

@@ -84,6 +84,6 @@ hipError_t launch_fill(const AtariStateGpu &tmpl, uint8_t *states, uint64_t *cpu
     return hipGetLastError();
 }
 
-void descriptor_table(uint32_t out[512]) { build_descriptors(out); }
+void descriptor_table(uint32_t out[kDescBytes / 4]) { build_descriptors(out); }
 
 }  // namespace fa

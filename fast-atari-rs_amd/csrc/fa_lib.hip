@@ -213,13 +213,13 @@ int fa_create(const uint8_t *rom, size_t rom_len, size_t n, fa_env **out) {
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(FA_EHIP, "hipStreamCreate failed"));
     if (fa::prepare_kernels(e->max_lds) != hipSuccess) return cleanup(fail(FA_EHIP, "kernel attribute setup failed"));
-    if (hipMalloc(&e->d_desc, 2048) != hipSuccess || hipMalloc(&e->d_rom, e->rom.rom_bytes) != hipSuccess ||
+    if (hipMalloc(&e->d_desc, fa::kDescBytes) != hipSuccess || hipMalloc(&e->d_rom, e->rom.rom_bytes) != hipSuccess ||
         hipMalloc(&e->d_states, e->n_pad * fa::kStateBytes) != hipSuccess ||
         hipMalloc(&e->d_cpu, e->n_pad * 8) != hipSuccess || hipMalloc(&e->d_fcyc, e->n_pad * 8) != hipSuccess ||
         hipMalloc(&e->d_instr, e->n_pad * 8) != hipSuccess || hipMalloc(&e->d_status, e->n_pad * 4) != hipSuccess ||
         hipMalloc(&e->d_obs, e->n_pad * 128) != hipSuccess || hipMalloc(&e->d_fs, e->n_pad) != hipSuccess)
         return cleanup(fail(FA_ENOMEM, "device allocation failed"));
-    uint32_t desc[512];
+    uint32_t desc[fa::kDescBytes / 4];
     fa::descriptor_table(desc);
     if (hipMemcpy(e->d_rom, e->rom.image.data(), e->rom.rom_bytes, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(e->d_desc, desc, sizeof(desc), hipMemcpyHostToDevice) != hipSuccess)

@@ -488,10 +488,10 @@ int fa_server_create(const uint8_t *const *roms, const size_t *rom_lens, size_t 
     if (fa::prepare_kernels(s->max_lds) != hipSuccess) return cleanup(fail(FA_EHIP, "kernel attribute setup failed"));
     if (hipStreamCreateWithFlags(&s->admin, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(FA_EHIP, "stream creation failed"));
-    if (hipMalloc(&s->d_desc, 2048) != hipSuccess ||
+    if (hipMalloc(&s->d_desc, fa::kDescBytes) != hipSuccess ||
         hipMalloc(&s->d_admin_tasks, 2 * (kMaxClientEnvs / kTaskEnvs) * 4) != hipSuccess)
         return cleanup(fail(FA_ENOMEM, "device allocation failed"));
-    uint32_t desc[512];
+    uint32_t desc[fa::kDescBytes / 4];
     fa::descriptor_table(desc);
     if (hipMemcpy(s->d_desc, desc, sizeof(desc), hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(FA_EHIP, "descriptor upload failed"));
