@@ -2,7 +2,8 @@
 //
 // Every documented NMOS 6502 opcode (151) is described by two 32-bit words
 // that steer one shared fetch / address / read / compute / write pipeline
-// with predicated selects instead of a per-opcode switch.  Cycle counts and
+// with predicated selects instead of a per-opcode switch, plus two words with
+// the same decode as one-hot masks for the per-lane path (16 bytes an opcode).  Cycle counts and
 // the order of bus accesses are those of the reference's Cpu::step
 // (src/cpu.rs:245-1146); the table itself is derived from the instruction
 // set (mnemonic x addressing mode), not from the reference's code.

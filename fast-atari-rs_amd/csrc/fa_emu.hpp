@@ -327,14 +327,23 @@ struct Emu {
     // touches the TIA.  Here the single-byte registers are one computed load
     // or one predicated store with selects; only the multi-byte ones (RESPx,
     // GRP0 / GRP1, HMOVE / HMCLR / CXCLR) branch, on a ballot.
-    __device__ __forceinline__ uint32_t tia_read_lanes(uint32_t addr) {
+    // LDS offset of the image byte behind TIA read register addr & 15: CXxx at
+    // S_CX + r (r < 8), paddles at S_PAD0 + r - 8, INPT4/5 at S_INPT4 + r - 12
+    // (registers 14 and 15 read as 0 and point anywhere in the image).
+    __device__ __forceinline__ uint32_t tia_read_off(uint32_t addr) const {
         const uint32_t r = addr & 0x0Fu;
-        // CXxx at S_CX + r (r < 8), paddles at S_PAD0 + r - 8, INPT4/5 at S_INPT4 + r - 12
         const uint32_t off = pick(r < 8u, S_CX + r, pick(r < 10u, S_PAD0 - 8u + r, S_INPT4 - 12u + r));
-        const uint32_t x = ld8(img + pick(r < 14u, off, S_CX));
+        return img + pick(r < 14u, off, S_CX);
+    }
+    // The register's value from that byte.
+    __device__ __forceinline__ uint32_t tia_read_val(uint32_t addr, uint32_t x) const {
+        const uint32_t r = addr & 0x0Fu;
         const uint32_t pad = pcount >= x ? 0x80u : 0u;
         const uint32_t inp = x ? 0x80u : 0u;
         return pick(r < 8u, x, pick(r < 10u, pad, pick(r < 12u, 0x80u, pick(r < 14u, inp, 0u))));
+    }
+    __device__ __forceinline__ uint32_t tia_read_lanes(uint32_t addr) {
+        return tia_read_val(addr, ld8(tia_read_off(addr)));
     }
     __device__ __forceinline__ void tia_write_lanes(uint32_t addr, uint32_t v) {
         const uint32_t r = addr & 0x3Fu;
@@ -548,11 +557,15 @@ struct Emu {
             const bool rom = (a13 & 0x1000u) != 0u;
             hotspot_if(rd, a13);
             const bool ram = (a13 & 0x1E80u) == 0x0080u;
+            // TIA registers are read without side effects: their byte is loaded
+            // beside the memory byte and selected; only the RIOT branches
+            const bool tia = (a13 & 0x1080u) == 0u;
             const uint32_t m = ld8(rd ? (rom ? ROM0 + (bank << 12) + (a13 & 0xFFFu) : ram_off(a13)) : junk);
-            v = rd ? m : v;
-            const bool io = rd & !rom & !ram;
-            if (__ballot(io) != 0) {
-                if (io) v = io_read(a13);
+            const uint32_t t = ld8(rd & tia ? tia_read_off(a13) : junk);
+            v = rd ? ((rom | ram) ? m : (tia ? tia_read_val(a13, t) : 0u)) : v;
+            const bool riot = rd & ((a13 & 0x1280u) == 0x0280u);
+            if (__ballot(riot) != 0) {
+                if (riot) v = pia_read(a13);
             }
         }
         // result candidates and the selected one
@@ -846,23 +859,33 @@ struct Emu {
         return true;
     }
 
-    // Fetch, advance PC and execute one instruction.  Returns the cycle
-    // count, or 0 for an illegal opcode.  Lanes whose PC is inside the ROM
-    // window (below the hotspot / vector area) fetch from LDS without side
-    // effects.  When the active lanes agree on the opcode, the wave takes
-    // the convergent path: a scalar dispatch on the opcode into the
-    // fast path specialised with that opcode's compile-time descriptor.
+    // The three ROM bytes at PC in the current bank (opcode and operands of
+    // the next instruction when PC is inside the ROM window; read ahead of
+    // exec() so their LDS latency overlaps other work).
+    __device__ __forceinline__ void fetch_ahead(uint32_t &op, uint32_t &b1, uint32_t &b2) const {
+        const uint32_t o = ROM0 + (bank << 12) + (pc & 0xFFFu);
+        op = ld8(o);
+        b1 = ld8(o + 1u);
+        b2 = ld8(o + 2u);
+    }
+
+    // Advance PC and execute one instruction, given fetch_ahead()'s bytes for
+    // the current PC and bank.  Returns the cycle count, or 0 for an illegal
+    // opcode.  Lanes whose PC is inside the ROM window (below the hotspot /
+    // vector area) use those bytes, as fetching there has no side effects;
+    // the others fetch through the bus.  When the active lanes agree on the
+    // opcode, the wave takes the convergent path: a scalar dispatch on the
+    // opcode into the fast path specialised with that opcode's compile-time
+    // descriptor.
     // With GROUP, lanes are first split into groups of equal opcode (64-bit
     // ballot + readfirstlane) and every group takes the convergent path.
     // Anything the fast path declines -- and divergent waves without GROUP --
     // goes through the general path (execute()), which replays the
     // reference's bus accesses one by one.
     template <bool GROUP>
-    __device__ __forceinline__ uint32_t exec() {
+    __device__ __forceinline__ uint32_t exec(uint32_t op, uint32_t b1, uint32_t b2) {
         const uint32_t pc0 = pc;
         const bool fz = (pc0 & 0x1000u) && ((pc0 & 0xFFFu) < 0xFF0u);
-        const uint32_t o = ROM0 + (bank << 12) + (pc0 & 0xFFFu);
-        const uint32_t op = ld8(o), b1 = ld8(o + 1u), b2 = ld8(o + 2u);
         const uint32_t key = fz ? op : 0x100u;
         uint32_t cyc = 0;
         bool done = false;

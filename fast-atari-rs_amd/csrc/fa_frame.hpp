@@ -27,6 +27,8 @@ __device__ __forceinline__ uint32_t run_frame(Emu &e, uint32_t &st, uint32_t &cp
     uint32_t cycles = 0;
     bool in_a = (e.vsync & 2u) != 0u;
     uint32_t hold = 0, backoff = 0;      // wave-uniform
+    uint32_t op, b1, b2;                 // ROM bytes at PC, read ahead
+    e.fetch_ahead(op, b1, b2);
     while (true) {
         if (hold == 0u) {
             if (u_maybe(e)) {
@@ -37,6 +39,7 @@ __device__ __forceinline__ uint32_t run_frame(Emu &e, uint32_t &st, uint32_t &cp
                     const bool moved = u.cycles != c0;
                     u_leave(u, e, cycles, in_a, cpu_cyc, instr);
                     if (r == U_DONE) break;
+                    e.fetch_ahead(op, b1, b2);
                     // declined at once: back off as after a failed test
                     backoff = moved ? 0u : (backoff ? (backoff < 64u ? 2u * backoff : 64u) : 1u);
                     hold = backoff;
@@ -56,8 +59,9 @@ __device__ __forceinline__ uint32_t run_frame(Emu &e, uint32_t &st, uint32_t &cp
             e.pia_tick(s);
             cycles += s;
         } else {
-            uint32_t c = e.exec<SORTED>();
+            uint32_t c = e.exec<SORTED>(op, b1, b2);
             if (c == 0u) { st = ENV_ILLEGAL; break; }
+            e.fetch_ahead(op, b1, b2);
             cycles += c;
             cpu_cyc += c;
             instr += 1u;
