@@ -12,7 +12,8 @@ The same line carries the heterogeneous batch ("mix": four ROM classes in
 one autotuned hipGraph launch of K = 8 steps with per-env frameskip) and the
 served mode ("served": 256 clients with continuous batching; p50 / p99 step
 latency beside aggregate frames/s; its bit-exactness, 1 % of envs against the
-oracle, is checked in tests/test_gpu_serve.py).
+oracle, is checked in tests/test_gpu_serve.py) and the env adapter
+("env_adapter": BatchAtariGpuEnv steps with f32 observations).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -127,6 +128,36 @@ def _pct(v, q):
     return round(float(np.percentile(v, q)), 3) if len(v) else None
 
 
+def bench_adapter(fastatari, rom, n, steps=16):
+    """The env adapter BatchAtariGpuEnv (src/rl4burn_env.rs:268-380) on the
+    same ROM and env count: one broadcast action per step held for the
+    default frameskip of 4 frames, f32 observations (RAM / 255) computed on
+    the GPU; API-level (obs copied to host memory every step) and with the
+    observations left in device memory.  frames/s = envs x 4 x steps / wall."""
+    env = fastatari.BatchAtariGpuEnv(rom, n)
+    env.reset()
+    for s in range(2):
+        env.step(s)
+    t0 = time.perf_counter()
+    for s in range(steps):
+        env.step(s % 18)
+    host_s = time.perf_counter() - t0
+    d_obs = fastatari.DeviceBuffer(n * 128 * 4)
+    env.step_device(d_obs.ptr, action=0)
+    env.sync()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        env.step_device(d_obs.ptr, action=s % 18)
+    env.sync()
+    dev_s = time.perf_counter() - t0
+    frames = n * 4 * steps
+    env.close()
+    d_obs.free()
+    return {"frameskip": 4, "steps": steps, "obs": "f32 RAM / 255 (n x 128)",
+            "api_fps": round(frames / host_s, 1), "device_obs_fps": round(frames / dev_s, 1),
+            "ms_per_step_api": round(host_s / steps * 1e3, 3), "ms_per_step_device": round(dev_s / steps * 1e3, 3)}
+
+
 def bench_serve(args, fastatari, W):
     """Serving: 256 clients on the four mixed ROMs (native load driver).  A
     closed-loop phase finds each client's saturation rate; then Poisson
@@ -207,6 +238,7 @@ def main():
     ap.add_argument("--no-mix", action="store_true", help="skip the heterogeneous-batch line")
     ap.add_argument("--mix-launches", type=int, default=3)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving line")
+    ap.add_argument("--no-adapter", action="store_true", help="skip the env-adapter line")
     ap.add_argument("--serve-seconds", type=float, default=6.0, help="open-loop serving time")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy kernel of MIB MiB (HBM counter calibration)")
@@ -328,6 +360,9 @@ def main():
     gpu.close()
     d_act.free()
     d_obs.free()
+    if not args.no_adapter:
+        log("bench: env adapter (broadcast action, frameskip 4, f32 obs)")
+        out["env_adapter"] = bench_adapter(fastatari, rom, n)
     if not args.no_mix:
         log("bench: heterogeneous batch (4 ROM classes, K=8, autotuned)")
         out["mix"] = bench_mix(args, fastatari, W)

@@ -83,6 +83,10 @@ hipError_t launch_fill(const AtariStateGpu &tmpl, uint8_t *states, uint64_t *cpu
                        uint32_t *status, uint32_t n, uint32_t n_pad, hipStream_t s);
 void descriptor_table(uint32_t out[kDescBytes / 4]);
 
+// fa_lib.hip: the batch handle's stream and device obs buffer (n * 128 B)
+hipStream_t env_stream(fa_env *e);
+uint8_t *env_obs_device(fa_env *e);
+
 // Errors: the message of the calling thread's last failure.
 int fail(int code, const std::string &msg);
 std::string &last_error();

@@ -186,6 +186,11 @@ int fill_power_on(fa_env *e) {
 
 }  // namespace
 
+namespace fa {
+hipStream_t env_stream(fa_env *e) { return e->stream; }
+uint8_t *env_obs_device(fa_env *e) { return e->d_obs; }
+}  // namespace fa
+
 extern "C" {
 
 const char *fa_last_error(void) { return fa::last_error().c_str(); }

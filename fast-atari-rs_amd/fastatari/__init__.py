@@ -8,7 +8,8 @@ built library or without a GPU raises.
 
 `BatchMixGpu` steps several ROM classes in one hipGraph launch (per-class
 launch shapes from its autotuner); `EnvServer` serves step requests of many
-clients with continuous batching.
+clients with continuous batching; `BatchAtariGpuEnv` is the env adapter over
+BatchAtariGpu (src/rl4burn_env.rs:268-380).
 """
 import os as _os
 
@@ -22,6 +23,7 @@ _os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 from .env import (STATE_DTYPE, BatchAtariGpu, Busy, FastAtariError, DeviceBuffer, lib_path, load_library)
 from .batch import BatchMixGpu
 from .serve import EnvServer
+from .vecenv import BatchAtariGpuEnv, Step
 
-__all__ = ["BatchAtariGpu", "BatchMixGpu", "EnvServer", "Busy", "FastAtariError", "DeviceBuffer", "STATE_DTYPE",
+__all__ = ["BatchAtariGpu", "BatchMixGpu", "EnvServer", "BatchAtariGpuEnv", "Step", "Busy", "FastAtariError", "DeviceBuffer", "STATE_DTYPE",
            "lib_path", "load_library"]

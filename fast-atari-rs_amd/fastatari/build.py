@@ -40,6 +40,7 @@ SOURCES = [
     ("fa_lib.hip", []),
     ("fa_batch.hip", []),
     ("fa_serve.hip", []),
+    ("fa_adapter.hip", []),
 ]
 
 

@@ -141,6 +141,22 @@ def load_library():
         "fa_server_download": (C.c_int, [vp, C.c_int, vp, u64p, u64p, u32p]),
         "fa_server_client_info": (C.c_int, [vp, C.c_int, C.POINTER(sz), C.POINTER(sz)]),
         "fa_server_set_class_config": (C.c_int, [vp, sz, C.c_uint32, C.c_int]),
+        "fa_vecenv_create": (C.c_int, [u8p, sz, sz, C.POINTER(vp)]),
+        "fa_vecenv_destroy": (None, [vp]),
+        "fa_vecenv_set_max_steps": (C.c_int, [vp, C.c_uint64]),
+        "fa_vecenv_set_frameskip": (C.c_int, [vp, C.c_uint32]),
+        "fa_vecenv_set_sorted": (C.c_int, [vp, C.c_int]),
+        "fa_vecenv_num_envs": (sz, [vp]),
+        "fa_vecenv_step_count": (C.c_uint64, [vp]),
+        "fa_vecenv_batch": (vp, [vp]),
+        "fa_vecenv_reset": (C.c_int, [vp, C.POINTER(C.c_float)]),
+        "fa_vecenv_step": (C.c_int, [vp, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "fa_vecenv_step_each": (C.c_int, [vp, u8p, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                          C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "fa_vecenv_step_device": (C.c_int, [vp, vp, C.c_uint32, vp, C.POINTER(C.c_int)]),
+        "fa_vecenv_sync": (C.c_int, [vp]),
+        "fa_obs_to_f32": (C.c_int, [vp, vp, sz]),
         "fa_server_memory": (C.c_int, [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]),
         "fa_server_set_coalesce_us": (C.c_int, [vp, C.c_uint32]),
         "fa_server_samples": (C.c_int, [vp, C.POINTER(C.c_double), u32p, u32p, u32p, sz, C.POINTER(sz), u64p,

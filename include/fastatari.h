@@ -251,6 +251,51 @@ int fa_loadgen_run(fa_server *s, const int *client_ids, const fa_client_spec *sp
                    uint64_t max_steps, uint64_t *steps, double *service_ms, double *e2e_ms, uint32_t *sample_cls,
                    size_t cap, size_t *nsamples, double *wall_s);
 
+/* ---- env adapter: BatchAtariGpuEnv (src/rl4burn_env.rs:268-380), the caller
+ * of BatchAtariGpu.  One action broadcast to every env and held for
+ * `frameskip` frames per step, observations = RAM / 255 as f32 (n * 128,
+ * normalised on the device), reward 0 and never terminated (the reference's
+ * constants), truncated once the step count reaches max_steps. */
+typedef struct fa_vecenv fa_vecenv;
+#define FA_NUM_ACTIONS 18
+#define FA_DEFAULT_MAX_STEPS 108000
+#define FA_DEFAULT_FRAMESKIP 4
+
+/* BatchAtariGpuEnv::new (src/rl4burn_env.rs:282-295): max_steps 108000,
+ * frameskip 4, unsorted. */
+int fa_vecenv_create(const uint8_t *rom, size_t rom_len, size_t n, fa_vecenv **out);
+void fa_vecenv_destroy(fa_vecenv *v);
+/* with_max_steps (0: never truncated), with_frameskip (>= 1), with_sorted
+ * (src/rl4burn_env.rs:297-313). */
+int fa_vecenv_set_max_steps(fa_vecenv *v, uint64_t max_steps);
+int fa_vecenv_set_frameskip(fa_vecenv *v, uint32_t frameskip);
+int fa_vecenv_set_sorted(fa_vecenv *v, int sorted);
+size_t fa_vecenv_num_envs(const fa_vecenv *v);
+/* The step count since the last reset. */
+uint64_t fa_vecenv_step_count(const fa_vecenv *v);
+/* The underlying batch (download_states, counters, ...). */
+fa_env *fa_vecenv_batch(fa_vecenv *v);
+/* Env::reset (src/rl4burn_env.rs:318-325): step count 0, every env to the
+ * ROM's reset state (BatchAtariGpu::reset); obs_out: n * 128 f32 (host). */
+int fa_vecenv_reset(fa_vecenv *v, float *obs_out);
+/* Env::step (src/rl4burn_env.rs:327-358): `action` (as u8; indices above 17
+ * are no-ops) for every env, held for frameskip frames in one launch; obs of
+ * the last frame (n * 128 f32, host; NULL: not copied).  reward / terminated /
+ * truncated may be NULL. */
+int fa_vecenv_step(fa_vecenv *v, uint32_t action, float *obs_out, float *reward, int *terminated, int *truncated);
+/* The same step with a per-env action (actions: n bytes, host). */
+int fa_vecenv_step_each(fa_vecenv *v, const uint8_t *actions, float *obs_out, float *reward, int *terminated,
+                        int *truncated);
+/* Device form: actions from device memory (d_actions: n bytes; NULL: `action`
+ * for every env), f32 obs into device memory d_obs (n * 128 floats); queued on
+ * the batch's stream, complete after fa_vecenv_sync.  Counts the step and
+ * reports truncation at once. */
+int fa_vecenv_step_device(fa_vecenv *v, const uint8_t *d_actions, uint32_t action, float *d_obs, int *truncated);
+int fa_vecenv_sync(fa_vecenv *v);
+/* u8 observations to f32 in [0, 1] (byte / 255, IEEE round-to-nearest), device
+ * memory to device memory, n_bytes values; synchronous. */
+int fa_obs_to_f32(const uint8_t *d_in, float *d_out, size_t n_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,6 +45,10 @@ def test_null_handles_are_errors_not_crashes():
     assert L.fa_num_envs(None) == 0
     assert L.fa_batch_num_envs(None) == 0
     assert L.fa_server_poll(None, 0) == -1
+    assert L.fa_vecenv_step(None, 0, None, None, None, None) == -1
+    assert L.fa_vecenv_num_envs(None) == 0
+    assert L.fa_vecenv_set_frameskip(None, 4) == -1
     L.fa_destroy(None)
     L.fa_batch_destroy(None)
     L.fa_server_destroy(None)
+    L.fa_vecenv_destroy(None)
