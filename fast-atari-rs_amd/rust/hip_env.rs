@@ -226,3 +226,101 @@ impl Drop for BatchMixGpu {
         unsafe { fa_batch_destroy(self.b) };
     }
 }
+
+// ---- env adapter: `BatchAtariGpuEnv` of src/rl4burn_env.rs:268-380 over the
+// fa_vecenv_* C ABI (f32 observations normalised on the GPU).  A maintainer
+// puts this beside the `cuda` variant under `#[cfg(feature = "hip")]`.
+
+use rl4burn_core::env::space::Space;
+use rl4burn_core::env::{Env, Step};
+
+#[repr(C)]
+pub struct FaVecEnv {
+    _private: [u8; 0],
+}
+
+extern "C" {
+    fn fa_vecenv_create(rom: *const u8, rom_len: usize, n: usize, out: *mut *mut FaVecEnv) -> c_int;
+    fn fa_vecenv_destroy(v: *mut FaVecEnv);
+    fn fa_vecenv_set_max_steps(v: *mut FaVecEnv, max_steps: u64) -> c_int;
+    fn fa_vecenv_set_frameskip(v: *mut FaVecEnv, frameskip: u32) -> c_int;
+    fn fa_vecenv_set_sorted(v: *mut FaVecEnv, sorted: c_int) -> c_int;
+    fn fa_vecenv_reset(v: *mut FaVecEnv, obs_out: *mut f32) -> c_int;
+    fn fa_vecenv_step(v: *mut FaVecEnv, action: u32, obs_out: *mut f32, reward: *mut f32, terminated: *mut c_int,
+                      truncated: *mut c_int) -> c_int;
+}
+
+/// N instances behind one `Env`: the action is broadcast to every instance
+/// and held for `frameskip` frames; observations are the 128 RAM bytes of
+/// each instance as f32 in [0, 1], flattened (`obs.len() == N * 128`).
+pub struct BatchAtariGpuEnv {
+    v: *mut FaVecEnv,
+    n: usize,
+}
+
+unsafe impl Send for BatchAtariGpuEnv {}
+
+impl BatchAtariGpuEnv {
+    /// Panics if HIP initialisation fails (as the CUDA variant does).
+    pub fn new(rom: Vec<u8>, n: usize) -> Self {
+        let mut v = std::ptr::null_mut();
+        check(unsafe { fa_vecenv_create(rom.as_ptr(), rom.len(), n, &mut v) })
+            .expect("failed to initialise HIP BatchAtariGpuEnv");
+        Self { v, n }
+    }
+
+    pub fn with_max_steps(self, max_steps: usize) -> Self {
+        check(unsafe { fa_vecenv_set_max_steps(self.v, max_steps as u64) }).expect("with_max_steps");
+        self
+    }
+
+    pub fn with_frameskip(self, frameskip: usize) -> Self {
+        check(unsafe { fa_vecenv_set_frameskip(self.v, frameskip as u32) }).expect("frameskip must be 1..255");
+        self
+    }
+
+    pub fn with_sorted(self, sorted: bool) -> Self {
+        check(unsafe { fa_vecenv_set_sorted(self.v, sorted as c_int) }).expect("with_sorted");
+        self
+    }
+
+    pub fn num_envs(&self) -> usize {
+        self.n
+    }
+}
+
+impl Env for BatchAtariGpuEnv {
+    type Observation = Vec<f32>;
+    type Action = usize;
+
+    fn reset(&mut self) -> Vec<f32> {
+        let mut obs = vec![0f32; self.n * 128];
+        check(unsafe { fa_vecenv_reset(self.v, obs.as_mut_ptr()) }).expect("HIP reset failed");
+        obs
+    }
+
+    fn step(&mut self, action: usize) -> Step<Vec<f32>> {
+        let mut obs = vec![0f32; self.n * 128];
+        let (mut reward, mut terminated, mut truncated) = (0f32, 0 as c_int, 0 as c_int);
+        check(unsafe {
+            fa_vecenv_step(self.v, action as u8 as u32, obs.as_mut_ptr(), &mut reward, &mut terminated, &mut truncated)
+        })
+        .expect("HIP step failed");
+        Step { observation: obs, reward, terminated: terminated != 0, truncated: truncated != 0 }
+    }
+
+    fn observation_space(&self) -> Space {
+        let size = self.n * 128;
+        Space::Box { low: vec![0.0; size], high: vec![1.0; size] }
+    }
+
+    fn action_space(&self) -> Space {
+        Space::Discrete(18)
+    }
+}
+
+impl Drop for BatchAtariGpuEnv {
+    fn drop(&mut self) {
+        unsafe { fa_vecenv_destroy(self.v) };
+    }
+}
