@@ -7,8 +7,14 @@
 // repeated (src/rl4burn_env.rs:339-353: set_action writes only input
 // registers the ROM cannot change, so setting it once or every frame is the
 // same).  The last frame's RAM becomes f32 observations on the device
-// (byte / 255, src/rl4burn_env.rs:314-316) before one copy to the host.
+// (byte / 255, src/rl4burn_env.rs:314-316) before one copy to the host.  An
+// optional RAM reader turns each env's RAM into a reward (score change) and a
+// terminated flag (lives field 0), also on the device; without one, reward
+// and terminated are the reference's constants 0 and false.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
 
 #include "fa_internal.hpp"
 
@@ -22,6 +28,14 @@ struct fa_vecenv {
     uint8_t *d_fs = nullptr;           // n: frameskip of every env
     uint32_t fs_held = 0;              // the value d_fs holds (0: not written yet)
     float *d_obs = nullptr;            // n * 128 f32 observations for host copies
+    // RAM reader (reward / termination)
+    bool reading = false;
+    fa_ram_reader reader{};
+    int64_t *d_score = nullptr;        // n: score after the last reset / step
+    float *d_reward = nullptr;         // n
+    uint8_t *d_term = nullptr;         // n
+    std::vector<float> h_reward;
+    std::vector<uint8_t> h_term;
 };
 
 namespace {
@@ -34,6 +48,32 @@ __global__ void __launch_bounds__(256) fa_obs_f32_kernel(const uint8_t *in, floa
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
         out[i] = __fdiv_rn((float)in[i], 255.0f);
+}
+
+// One env per lane: score from the RAM bytes (BCD or binary, most significant
+// first), reward = score - previous score (0 when `first`), terminated when
+// the masked lives field is 0.
+__global__ void __launch_bounds__(256) fa_reader_kernel(const uint8_t *obs8, uint32_t n, fa_ram_reader r,
+                                                        int first, int64_t *score, float *reward, uint8_t *term) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint8_t *ram = obs8 + (size_t)i * FA_OBS_BYTES;
+    int64_t s = 0;
+    for (uint32_t k = 0; k < r.n_score; ++k) {
+        const uint32_t b = ram[r.score_addr[k] & 0x7Fu];
+        s = r.score_bcd ? s * 100 + (int64_t)((b >> 4) * 10u + (b & 15u)) : s * 256 + (int64_t)b;
+    }
+    reward[i] = first ? 0.0f : (float)(s - score[i]);
+    score[i] = s;
+    const uint32_t lives = ((uint32_t)ram[r.lives_addr & 0x7Fu] >> (r.lives_shift & 7u)) & r.lives_mask;
+    term[i] = (r.lives_mask != 0 && lives == 0) ? 1 : 0;
+}
+
+hipError_t launch_reader(fa_vecenv *v, int first) {
+    hipLaunchKernelGGL(fa_reader_kernel, dim3((unsigned)((v->n + 255) / 256)), dim3(256), 0, fa::env_stream(v->env),
+                       (const uint8_t *)fa::env_obs_device(v->env), (uint32_t)v->n, v->reader, first, v->d_score,
+                       v->d_reward, v->d_term);
+    return hipGetLastError();
 }
 
 hipError_t launch_obs_f32(const uint8_t *in, float *out, size_t n, hipStream_t s) {
@@ -55,13 +95,25 @@ int launch_step(fa_vecenv *v, const uint8_t *d_act, float *d_out) {
     int rc = fa_step_frameskip_device(v->env, d_act, v->d_fs, obs8);
     if (rc) return rc;
     FA_HIP_TRY(launch_obs_f32(obs8, d_out, v->n * FA_OBS_BYTES, s));
+    if (v->reading) FA_HIP_TRY(launch_reader(v, 0));
     return FA_OK;
 }
 
+// Step count and truncation; reward / terminated: the reference's 0 / false,
+// or with a reader the sum of the per-env rewards and whether every env is
+// terminated (from the host copies finish_host made).
 void count_step(fa_vecenv *v, float *reward, int *terminated, int *truncated) {
     v->steps += 1;
-    if (reward) *reward = 0.0f;              // src/rl4burn_env.rs:347-352
-    if (terminated) *terminated = 0;
+    double sum = 0.0;
+    bool all = v->reading;
+    if (v->reading && (reward || terminated)) {
+        for (size_t i = 0; i < v->n; ++i) {
+            sum += v->h_reward[i];
+            all = all && v->h_term[i];
+        }
+    }
+    if (reward) *reward = (float)sum;        // src/rl4burn_env.rs:347-352 without a reader
+    if (terminated) *terminated = all ? 1 : 0;
     if (truncated) *truncated = (v->max_steps > 0 && v->steps >= v->max_steps) ? 1 : 0;
 }
 
@@ -69,6 +121,10 @@ int finish_host(fa_vecenv *v, float *obs_out) {
     hipStream_t s = fa::env_stream(v->env);
     if (obs_out)
         FA_HIP_TRY(hipMemcpyAsync(obs_out, v->d_obs, v->n * FA_OBS_BYTES * sizeof(float), hipMemcpyDeviceToHost, s));
+    if (v->reading) {
+        FA_HIP_TRY(hipMemcpyAsync(v->h_reward.data(), v->d_reward, v->n * sizeof(float), hipMemcpyDeviceToHost, s));
+        FA_HIP_TRY(hipMemcpyAsync(v->h_term.data(), v->d_term, v->n, hipMemcpyDeviceToHost, s));
+    }
     FA_HIP_TRY(hipStreamSynchronize(s));
     return FA_OK;
 }
@@ -100,7 +156,8 @@ int fa_vecenv_create(const uint8_t *rom, size_t rom_len, size_t n, fa_vecenv **o
 void fa_vecenv_destroy(fa_vecenv *v) {
     if (!v) return;
     if (v->env) hipStreamSynchronize(fa::env_stream(v->env));
-    for (void *p : {(void *)v->d_actions, (void *)v->d_fs, (void *)v->d_obs})
+    for (void *p : {(void *)v->d_actions, (void *)v->d_fs, (void *)v->d_obs, (void *)v->d_score,
+                    (void *)v->d_reward, (void *)v->d_term})
         if (p) hipFree(p);
     fa_destroy(v->env);
     delete v;
@@ -136,6 +193,7 @@ int fa_vecenv_reset(fa_vecenv *v, float *obs_out) {
     int rc = fa_reset(v->env, nullptr);
     if (rc) return rc;
     FA_HIP_TRY(launch_obs_f32(fa::env_obs_device(v->env), v->d_obs, v->n * FA_OBS_BYTES, fa::env_stream(v->env)));
+    if (v->reading) FA_HIP_TRY(launch_reader(v, 1));
     return finish_host(v, obs_out);
 }
 
@@ -179,6 +237,44 @@ int fa_vecenv_step_device(fa_vecenv *v, const uint8_t *d_actions, uint32_t actio
 int fa_vecenv_sync(fa_vecenv *v) {
     if (!v) return fail(FA_EINVAL, "NULL handle");
     return fa_sync(v->env);
+}
+
+int fa_vecenv_set_reader(fa_vecenv *v, const fa_ram_reader *r) {
+    if (!v) return fail(FA_EINVAL, "NULL handle");
+    if (!r) {
+        v->reading = false;
+        return FA_OK;
+    }
+    if (r->n_score > 4) return fail(FA_EINVAL, "n_score must be 0..4");
+    if (r->lives_shift > 7) return fail(FA_EINVAL, "lives_shift must be 0..7");
+    for (uint32_t k = 0; k < r->n_score; ++k)
+        if (r->score_addr[k] > 127) return fail(FA_EINVAL, "score_addr must be 0..127 (RAM offsets)");
+    if (r->lives_addr > 127) return fail(FA_EINVAL, "lives_addr must be 0..127 (RAM offset)");
+    if (!v->d_score) {
+        if (hipMalloc(&v->d_score, v->n * sizeof(int64_t)) != hipSuccess ||
+            hipMalloc(&v->d_reward, v->n * sizeof(float)) != hipSuccess || hipMalloc(&v->d_term, v->n) != hipSuccess)
+            return fail(FA_ENOMEM, "device allocation failed");
+        v->h_reward.assign(v->n, 0.0f);
+        v->h_term.assign(v->n, 0);
+    }
+    v->reader = *r;
+    v->reading = true;
+    FA_HIP_TRY(launch_reader(v, 1));
+    return finish_host(v, nullptr);
+}
+
+int fa_vecenv_rewards(fa_vecenv *v, float *rewards, uint8_t *terminated) {
+    if (!v) return fail(FA_EINVAL, "NULL handle");
+    hipStream_t s = fa::env_stream(v->env);
+    if (!v->reading) {
+        if (rewards) std::fill(rewards, rewards + v->n, 0.0f);
+        if (terminated) std::fill(terminated, terminated + v->n, (uint8_t)0);
+        return FA_OK;
+    }
+    if (rewards) FA_HIP_TRY(hipMemcpyAsync(rewards, v->d_reward, v->n * sizeof(float), hipMemcpyDeviceToHost, s));
+    if (terminated) FA_HIP_TRY(hipMemcpyAsync(terminated, v->d_term, v->n, hipMemcpyDeviceToHost, s));
+    FA_HIP_TRY(hipStreamSynchronize(s));
+    return FA_OK;
 }
 
 int fa_obs_to_f32(const uint8_t *d_in, float *d_out, size_t n_bytes) {

@@ -157,6 +157,8 @@ def load_library():
         "fa_vecenv_step_device": (C.c_int, [vp, vp, C.c_uint32, vp, C.POINTER(C.c_int)]),
         "fa_vecenv_sync": (C.c_int, [vp]),
         "fa_obs_to_f32": (C.c_int, [vp, vp, sz]),
+        "fa_vecenv_set_reader": (C.c_int, [vp, vp]),
+        "fa_vecenv_rewards": (C.c_int, [vp, C.POINTER(C.c_float), u8p]),
         "fa_server_memory": (C.c_int, [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]),
         "fa_server_set_coalesce_us": (C.c_int, [vp, C.c_uint32]),
         "fa_server_samples": (C.c_int, [vp, C.POINTER(C.c_double), u32p, u32p, u32p, sz, C.POINTER(sz), u64p,

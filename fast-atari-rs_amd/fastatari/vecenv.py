@@ -8,7 +8,8 @@ in [0, 1] (byte / 255, computed on the GPU), flattened to n * 128 values;
 reward is 0 and `terminated` False (the reference's constants); `truncated`
 once the step count reaches max_steps (0: never).  `step_each` takes one
 action per env; `step_device` keeps actions and f32 observations in device
-memory.
+memory.  `with_reader` adds per-env rewards and termination read from RAM on
+the GPU (the reference leaves both to game-specific wrappers).
 """
 from __future__ import annotations
 
@@ -22,6 +23,14 @@ from .env import BatchAtariGpu, _check, _p8, _u8, load_library
 NUM_ACTIONS = 18                 # src/rl4burn_env.rs:26
 DEFAULT_MAX_STEPS = 108_000      # src/rl4burn_env.rs:29
 DEFAULT_FRAMESKIP = 4            # src/rl4burn_env.rs:32
+
+
+class RamReader(C.Structure):
+    """fa_ram_reader: score bytes (RAM offsets, most significant first; BCD or
+    binary) and a lives field; see include/fastatari.h."""
+    _fields_ = [("score_addr", C.c_uint8 * 4), ("n_score", C.c_uint8), ("score_bcd", C.c_uint8),
+                ("lives_addr", C.c_uint8), ("lives_shift", C.c_uint8), ("lives_mask", C.c_uint8),
+                ("_pad", C.c_uint8 * 3)]
 
 
 class Step(NamedTuple):
@@ -68,6 +77,34 @@ class BatchAtariGpuEnv:
     def with_sorted(self, sorted_: bool) -> "BatchAtariGpuEnv":
         _check(load_library().fa_vecenv_set_sorted(self._h, int(bool(sorted_))))
         return self
+
+    def with_reader(self, score_addrs=(), bcd: bool = False, lives_addr: int = 0, lives_shift: int = 0,
+                    lives_mask: int = 0) -> "BatchAtariGpuEnv":
+        """Reward = change of the score held in RAM bytes `score_addrs` (offsets
+        0..127 into the RAM observation, most significant first; packed BCD
+        when `bcd`); terminated while (ram[lives_addr] >> lives_shift) &
+        lives_mask is 0 (mask 0: never).  Evaluated on the GPU."""
+        r = RamReader()
+        addrs = list(score_addrs)
+        if len(addrs) > 4:
+            raise ValueError("at most 4 score bytes")
+        for k, a in enumerate(addrs):
+            r.score_addr[k] = int(a)
+        r.n_score, r.score_bcd = len(addrs), int(bool(bcd))
+        r.lives_addr, r.lives_shift, r.lives_mask = int(lives_addr), int(lives_shift), int(lives_mask)
+        _check(load_library().fa_vecenv_set_reader(self._h, C.byref(r)))
+        return self
+
+    def without_reader(self) -> "BatchAtariGpuEnv":
+        _check(load_library().fa_vecenv_set_reader(self._h, None))
+        return self
+
+    def rewards(self) -> tuple[np.ndarray, np.ndarray]:
+        """Per-env reward (f32) and terminated flag of the last reset / step."""
+        rew = np.empty(self.n, np.float32)
+        term = np.empty(self.n, np.uint8)
+        _check(load_library().fa_vecenv_rewards(self._h, _fp(rew), _p8(term)))
+        return rew, term.astype(bool)
 
     def num_envs(self) -> int:
         return int(load_library().fa_vecenv_num_envs(self._h))

@@ -248,6 +248,22 @@ extern "C" {
     fn fa_vecenv_reset(v: *mut FaVecEnv, obs_out: *mut f32) -> c_int;
     fn fa_vecenv_step(v: *mut FaVecEnv, action: u32, obs_out: *mut f32, reward: *mut f32, terminated: *mut c_int,
                       truncated: *mut c_int) -> c_int;
+    fn fa_vecenv_set_reader(v: *mut FaVecEnv, reader: *const FaRamReader) -> c_int;
+    fn fa_vecenv_rewards(v: *mut FaVecEnv, rewards: *mut f32, terminated: *mut u8) -> c_int;
+}
+
+/// Reward / termination reader over RAM (fa_ram_reader): score bytes (RAM
+/// offsets, most significant first; packed BCD or binary) and a lives field.
+#[repr(C)]
+#[derive(Clone, Copy, Debug, Default)]
+pub struct FaRamReader {
+    pub score_addr: [u8; 4],
+    pub n_score: u8,
+    pub score_bcd: u8,
+    pub lives_addr: u8,
+    pub lives_shift: u8,
+    pub lives_mask: u8,
+    pub _pad: [u8; 3],
 }
 
 /// N instances behind one `Env`: the action is broadcast to every instance
@@ -286,6 +302,21 @@ impl BatchAtariGpuEnv {
 
     pub fn num_envs(&self) -> usize {
         self.n
+    }
+
+    /// Per-env rewards and termination read from RAM on the GPU (the
+    /// reference leaves both to game-specific wrappers); `step` then returns
+    /// their sum and whether every instance is terminated.
+    pub fn with_reader(self, reader: FaRamReader) -> Self {
+        check(unsafe { fa_vecenv_set_reader(self.v, &reader) }).expect("invalid RAM reader");
+        self
+    }
+
+    /// Per-instance reward and terminated flag of the last reset / step.
+    pub fn rewards(&mut self) -> (Vec<f32>, Vec<bool>) {
+        let (mut r, mut t) = (vec![0f32; self.n], vec![0u8; self.n]);
+        check(unsafe { fa_vecenv_rewards(self.v, r.as_mut_ptr(), t.as_mut_ptr()) }).expect("rewards");
+        (r, t.into_iter().map(|x| x != 0).collect())
     }
 }
 

@@ -296,6 +296,29 @@ int fa_vecenv_sync(fa_vecenv *v);
  * memory to device memory, n_bytes values; synchronous. */
 int fa_obs_to_f32(const uint8_t *d_in, float *d_out, size_t n_bytes);
 
+/* Reward / termination readers over RAM.  The reference leaves both to
+ * game-specific wrappers and returns 0 / false (src/rl4burn_env.rs:347-352,
+ * src/atari.rs:347-352).  Score: the RAM bytes score_addr[0 .. n_score)
+ * (offsets 0..127 into the 128 RAM bytes, most significant first), packed BCD
+ * (two decimal digits a byte) when score_bcd, else binary; an env's reward
+ * for a step is its score after the step minus its score before, as f32.
+ * Lives: (ram[lives_addr] >> lives_shift) & lives_mask; an env is terminated
+ * while that is 0 (lives_mask 0: never).  Evaluated on the GPU after reset and
+ * after every step; with a reader set, fa_vecenv_step's reward is the sum over
+ * envs and terminated is 1 when every env is terminated. */
+typedef struct fa_ram_reader {
+    uint8_t score_addr[4];
+    uint8_t n_score;                   /* 0..4; 0: every reward 0 */
+    uint8_t score_bcd;
+    uint8_t lives_addr, lives_shift, lives_mask;
+    uint8_t _pad[3];
+} fa_ram_reader;
+/* reader NULL: none (reward 0, never terminated).  Scores start from the
+ * current RAM (reset sets them again). */
+int fa_vecenv_set_reader(fa_vecenv *v, const fa_ram_reader *reader);
+/* Per-env reward and terminated flag of the last reset / step (host, n each). */
+int fa_vecenv_rewards(fa_vecenv *v, float *rewards, uint8_t *terminated);
+
 #ifdef __cplusplus
 }
 #endif

@@ -124,3 +124,53 @@ def test_device_step_and_sorted_match_host(fa, W):
     assert np.array_equal(d_obs.download().view(np.float32), want)
     assert dev.action_space() == {"kind": "discrete", "n": 18}
     assert dev.observation_space()["shape"] == (n * 128,)
+
+
+def _score(ram: np.ndarray, addrs, bcd: bool) -> np.ndarray:
+    s = np.zeros(ram.shape[0], np.int64)
+    for a in addrs:
+        b = ram[:, a].astype(np.int64)
+        s = s * 100 + (b >> 4) * 10 + (b & 15) if bcd else s * 256 + b
+    return s
+
+
+@pytest.mark.parametrize("bcd", [False, True])
+def test_ram_reader_rewards_and_termination(fa, orc, W, bcd):
+    """Per-env reward = change of a multi-byte score in RAM (binary or packed
+    BCD), terminated = a masked lives field at 0, against the same reading of
+    the oracle's RAM; the batch reward is their sum, terminated their AND."""
+    rom = W.rom_for(1)
+    n, fs = 256, 2
+    addrs, lives_addr, lives_shift, lives_mask = [0x00, 0x01, 0x05], 0x00, 4, 3
+    env = (fa.BatchAtariGpuEnv(rom, n).with_frameskip(fs)
+           .with_reader(addrs, bcd=bcd, lives_addr=lives_addr, lives_shift=lives_shift, lives_mask=lives_mask))
+    ref = orc.Batch(rom, n, nthreads=16)
+    env.reset()
+    ref.reset()
+    ram = ref.obs().reshape(n, 128)
+    prev = _score(ram, addrs, bcd)
+    rew, term = env.rewards()
+    assert np.all(rew == 0.0)
+    assert np.array_equal(term, ((ram[:, lives_addr] >> lives_shift) & lives_mask) == 0)
+    acts = W.ActionStream(W.config_seed(1), n)
+    nonzero = 0
+    for s in range(6):
+        a = acts.next()
+        st = env.step_each(a)
+        ref.run(np.repeat(a[None, :], fs, axis=0))
+        ram = ref.obs().reshape(n, 128)
+        sc = _score(ram, addrs, bcd)
+        want = (sc - prev).astype(np.float32)
+        prev = sc
+        rew, term = env.rewards()
+        assert _same_bits(rew, want), f"step {s} rewards"
+        wterm = ((ram[:, lives_addr] >> lives_shift) & lives_mask) == 0
+        assert np.array_equal(term, wterm), f"step {s} terminated"
+        assert 0 < wterm.sum() < n
+        assert st.reward == np.float32(rew.astype(np.float64).sum())
+        assert st.terminated == bool(wterm.all())
+        nonzero += int((rew != 0).sum())
+    assert nonzero > 0
+    env.without_reader()
+    st = env.step(0)
+    assert st.reward == 0.0 and st.terminated is False
