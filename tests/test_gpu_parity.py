@@ -101,6 +101,32 @@ def test_config3_branchy(fa, orc, W, sorted_):
         assert_same(gpu, ref, f"frame {f}")
 
 
+@pytest.mark.parametrize("cfg,frames,k", [(2, 1000, 16), (3, 300, 10)])
+def test_full_size_sampled(fa, orc, W, cfg, frames, k):
+    """Configs 2 and 3 at their full size (65,536 envs; 1,000 F8 frames as
+    step_multi K = 16, 300 branchy frames), every env on the GPU and a seeded
+    sample of 1,024 envs replayed on the oracle with the same action columns:
+    state, counters and status equal half-way and at the end."""
+    rom = W.rom_for(cfg)
+    n = 65536
+    gpu = fa.BatchAtariGpu(rom, n)
+    gpu.reset()
+    acts = W.ActionStream(W.config_seed(cfg), n).frames(frames)       # [frames, n]
+    sample = np.sort(np.random.default_rng(cfg).choice(n, 1024, replace=False))
+    ref = orc.Batch(rom, len(sample), nthreads=16)
+    ref.reset()
+    d_act, d_obs = fa.DeviceBuffer(acts.nbytes), fa.DeviceBuffer(n * 128)
+    d_act.upload(acts)
+    half = (frames // 2 // k) * k
+    for lo, hi in ((0, half), (half, frames)):
+        for f in range(lo, hi, k):
+            gpu.step_device(d_act.at(f * n), min(k, hi - f), d_obs.ptr)
+        gpu.sync()
+        ref.run(acts[lo:hi, sample])
+        assert_same(gpu, ref, f"config {cfg} after frame {hi}", env_ids=sample, gpu_rows=sample)
+        assert np.array_equal(d_obs.download().reshape(n, 128)[sample], ref.obs())
+
+
 @pytest.mark.parametrize("case", ["identical", "few_differ"])
 def test_branchy_uniform_waves(fa, orc, W, case):
     """The branchy ROM -- every documented opcode: JSR/RTS, BRK/RTI, JMP (ind),
