@@ -29,6 +29,7 @@ struct UCtl {
     uint32_t pc, sp, bank;
     uint32_t clock, line, wsync, fcomp, vsync;
     uint32_t tval, tps, tsel, tsh, tuf;
+    uint32_t tlim;           // timer fast-path budget (u_tlim)
     uint32_t cycles, in_a;   // run_frame's cycle count and loop-A flag
     uint32_t cpu, instr;     // CPU cycles and instructions executed since entry
 };
@@ -118,13 +119,23 @@ __device__ __forceinline__ uint32_t u_tia_skip(UCtl &u, Emu &e) {
     return cyc;
 }
 
+// Cycles a timer advance may take by the fast path alone (prescaler count
+// minus the cycles, no value change): the prescaler count, or 0 in the
+// post-underflow state (value 0 with the underflow flag set), where every
+// cycle decrements the value.  Kept in step with tval / tps / tuf, so the
+// per-instruction test is one compare.
+__device__ __forceinline__ uint32_t u_tlim(const UCtl &u) { return (u.tval == 0u && u.tuf) ? 0u : u.tps; }
+
 // ---------------------------------------------------------------- bus
 // Side effects of a read at a wave-uniform 13-bit address whose value is not
 // used: a hotspot switches the bank, INTIM clears the underflow flag.
 __device__ __forceinline__ void u_touch(UCtl &u, const Emu &e, uint32_t a13) {
     const uint32_t hd = (a13 & 0xFFFu) - e.hot_lo;
     if ((a13 & 0x1000u) && hd < e.hot_n) u.bank = hd;
-    if ((a13 & 0x1287u) == 0x284u) u.tuf = 0;
+    if ((a13 & 0x1287u) == 0x284u) {
+        u.tuf = 0;
+        u.tlim = u.tps;
+    }
 }
 // ROM read at a uniform address (HeadlessBus::read: the hotspot first).
 __device__ __forceinline__ uint32_t u_rom_read(UCtl &u, const Emu &e, uint32_t a13) {
@@ -135,7 +146,7 @@ __device__ __forceinline__ uint32_t u_rom_read(UCtl &u, const Emu &e, uint32_t a
 __device__ __forceinline__ uint32_t u_io_read(UCtl &u, Emu &e, uint32_t a13) {
     if ((a13 & 0x280u) == 0x280u) {
         const uint32_t r = a13 & 7u;
-        if (r == 4u) { u.tuf = 0; return u.tval & 0xFFu; }     // INTIM
+        if (r == 4u) { u.tuf = 0; u.tlim = u.tps; return u.tval & 0xFFu; }   // INTIM
         if (r == 5u) return u.tuf ? 0x80u : 0u;                 // INSTAT
         return e.pia_read(a13);                                 // ports: per lane
     }
@@ -146,8 +157,12 @@ __device__ __forceinline__ uint32_t u_io_read(UCtl &u, Emu &e, uint32_t a13) {
 // has checked that VSYNC and timer writes carry a wave-uniform value.
 __device__ __forceinline__ void u_io_write(UCtl &u, Emu &e, uint32_t a13, uint32_t v) {
     if (a13 & 0x200u) {
-        if (a13 & 0x10u) timer_set(u, a13, first_lane(v));
-        else e.pia_write(a13, v);
+        if (a13 & 0x10u) {
+            timer_set(u, a13, first_lane(v));
+            u.tlim = u.tps;
+        } else {
+            e.pia_write(a13, v);
+        }
         return;
     }
     const uint32_t r = a13 & 0x3Fu;
@@ -498,6 +513,7 @@ __device__ __forceinline__ uint32_t u_run(UCtl &u, Emu &e, uint32_t &st) {
     u.fcomp = wave_first(u.fcomp); u.vsync = wave_first(u.vsync);
     u.tval = wave_first(u.tval); u.tps = wave_first(u.tps); u.tsel = wave_first(u.tsel);
     u.tsh = wave_first(u.tsh); u.tuf = wave_first(u.tuf);
+    u.tlim = u_tlim(u);
     u.cycles = wave_first(u.cycles); u.in_a = wave_first(u.in_a);
     u.cpu = 0;
     u.instr = 0;
@@ -521,6 +537,7 @@ __device__ __forceinline__ uint32_t u_run(UCtl &u, Emu &e, uint32_t &st) {
             const uint32_t s = u_tia_skip(u, e);
             if (s == 0u) { st = ENV_VSYNC_HELD; return U_DONE; }
             timer_tick(u, s);
+            u.tlim = u_tlim(u);
             u.cycles += s;
         } else {
             // opcode and operands straight from the LDS ROM image: below the
@@ -543,8 +560,13 @@ __device__ __forceinline__ uint32_t u_run(UCtl &u, Emu &e, uint32_t &st) {
             u.instr += 1u;
             u_tia_tick(u, e, cyc);
             // Pia::tick_n's fast branch inline, the rest out of line
-            if (__builtin_expect(u.tps > cyc, 1) && (u.tval != 0u || !u.tuf)) u.tps -= cyc;
-            else timer_tick_slow(u, cyc);
+            if (__builtin_expect(u.tlim > cyc, 1)) {
+                u.tps -= cyc;
+                u.tlim -= cyc;
+            } else {
+                timer_tick_slow(u, cyc);
+                u.tlim = u_tlim(u);
+            }
             const uint32_t o = ROM0 + (u.bank << 12) + (u.pc & 0xFFFu);
             f0 = ld8(o); f1 = ld8(o + 1u); f2 = ld8(o + 2u);
         }
