@@ -27,7 +27,7 @@ namespace fa {
 // Wave-uniform control state (scalar registers).
 struct UCtl {
     uint32_t pc, sp, bank;
-    uint32_t clock, line, wsync, fcomp, vsync;
+    uint32_t clock, line, wsync, fcomp, vsync;   // wsync bit 1: see u_loop_bit
     uint32_t tval, tps, tsel, tsh, tuf;
     uint32_t tlim;           // timer fast-path budget (u_tlim)
     uint32_t cycles, in_a;   // run_frame's cycle count and loop-A flag
@@ -53,6 +53,13 @@ __device__ __forceinline__ bool same_on_all(uint32_t v, uint32_t s) { return __b
 __device__ __forceinline__ bool u_maybe(const Emu &e) {
     const uint32_t w = e.pc | (e.clock << 16);
     return same_on_all(w, wave_first(w));
+}
+
+// Bit 1 of UCtl::wsync: VSYNC no longer agrees with the frame loop (loop A
+// runs while VSYNC is held, loop B until it rises), set when VSYNC is written
+// and at entry, so the loop tests one word for both this and WSYNC.
+__device__ __forceinline__ uint32_t u_loop_bit(const UCtl &u) {
+    return (((u.vsync >> 1) & 1u) != u.in_a) ? 2u : 0u;
 }
 
 // Full test; on success fills u from the common state.  Fields are compared at
@@ -82,6 +89,7 @@ __device__ __forceinline__ bool u_enter(const Emu &e, uint32_t cycles, bool in_a
     u.vsync = (s4 >> 8) & 0xFFu;
     u.in_a = (s4 >> 16) & 1u;
     u.cycles = s5;
+    u.wsync |= u_loop_bit(u);
     u.cpu = 0;
     u.instr = 0;
     return true;
@@ -90,7 +98,7 @@ __device__ __forceinline__ bool u_enter(const Emu &e, uint32_t cycles, bool in_a
 __device__ __forceinline__ void u_leave(const UCtl &u, Emu &e, uint32_t &cycles, bool &in_a, uint32_t &cpu_cyc,
                                         uint32_t &instr) {
     e.pc = u.pc; e.sp = u.sp; e.bank = u.bank;
-    e.clock = u.clock; e.line = u.line; e.wsync = u.wsync; e.fcomp = u.fcomp; e.vsync = u.vsync;
+    e.clock = u.clock; e.line = u.line; e.wsync = u.wsync & 1u; e.fcomp = u.fcomp; e.vsync = u.vsync;
     e.tval = u.tval; e.tps = u.tps; e.tsel = u.tsel; e.tsh = u.tsh; e.tuf = u.tuf;
     cycles = u.cycles;
     in_a = u.in_a != 0u;
@@ -104,7 +112,7 @@ __device__ __forceinline__ void u_leave(const UCtl &u, Emu &e, uint32_t &cycles,
 __device__ __forceinline__ void u_end_scanline(UCtl &u, Emu &e) {
     u.clock -= 228u;
     u.line += 1u;
-    u.wsync = 0;
+    u.wsync &= 2u;
     e.pcount += (uint32_t)(e.pdump == 0u) & (uint32_t)(e.pcount < 256u);
     if (u.line >= 262u) { u.line = 0; u.fcomp = 1; }
 }
@@ -170,6 +178,7 @@ __device__ __forceinline__ void u_io_write(UCtl &u, Emu &e, uint32_t a13, uint32
         const uint32_t vu = first_lane(v);
         if (!(u.vsync & 2u) && (vu & 2u)) { u.line = 0; u.fcomp = 1; }
         u.vsync = vu;
+        u.wsync = (u.wsync & 1u) | u_loop_bit(u);
     } else if (r == 0x01u) {
         // VBLANK: bit 7 dumps the paddle capacitor (src/headless_tia.rs:232-240)
         const bool dump = (v & 0x80u) != 0u;
@@ -177,7 +186,7 @@ __device__ __forceinline__ void u_io_write(UCtl &u, Emu &e, uint32_t a13, uint32
         e.pdump = dump ? 1u : 0u;
         st8(e.img + S_VBLANK, v);
     } else if (r == 0x02u) {
-        u.wsync = 1;
+        u.wsync |= 1u;
     } else if (r >= 0x10u && r <= 0x14u) {
         st16(e.img + S_POS + 2u * (r - 0x10u), resp_pos_of(u.clock));
     } else {
@@ -526,20 +535,25 @@ __device__ __forceinline__ uint32_t u_run(UCtl &u, Emu &e, uint32_t &st) {
         f0 = ld8(o); f1 = ld8(o + 1u); f2 = ld8(o + 2u);
     }
     while (true) {
-        // loop A runs while VSYNC is held and leaves when it drops; loop B
-        // ends the frame when VSYNC rises
-        const uint32_t v2 = (u.vsync >> 1) & 1u;
-        if (__builtin_expect(u.in_a != v2, 0)) {
-            if (!u.in_a) return U_DONE;
-            u.in_a = 0;
+        // one test for two rare events: WSYNC (bit 0), and VSYNC disagreeing
+        // with the loop (bit 1): loop A runs while VSYNC is held and leaves
+        // when it drops; loop B ends the frame when VSYNC rises
+        if (__builtin_expect(u.wsync != 0u, 0)) {
+            if (u.wsync & 2u) {
+                if (!u.in_a) return U_DONE;            // VSYNC rose in loop B: the frame ends
+                u.in_a = 0;                            // VSYNC dropped: loop A is over
+                u.wsync &= 1u;
+            }
+            if (u.wsync) {
+                const uint32_t s = u_tia_skip(u, e);
+                if (s == 0u) { st = ENV_VSYNC_HELD; return U_DONE; }
+                timer_tick(u, s);
+                u.tlim = u_tlim(u);
+                u.cycles += s;
+                goto caps;
+            }
         }
-        if (u.wsync) {
-            const uint32_t s = u_tia_skip(u, e);
-            if (s == 0u) { st = ENV_VSYNC_HELD; return U_DONE; }
-            timer_tick(u, s);
-            u.tlim = u_tlim(u);
-            u.cycles += s;
-        } else {
+        {
             // opcode and operands straight from the LDS ROM image: below the
             // hotspot / vector area these fetches have no side effects
             if (__builtin_expect(((u.pc & 0x1FFFu) - 0x1000u) >= 0xFF0u, 0)) return U_EXIT;
@@ -572,6 +586,7 @@ __device__ __forceinline__ uint32_t u_run(UCtl &u, Emu &e, uint32_t &st) {
         }
         // the frame-loop caps (loop A's is the larger)
         static_assert(LOOPA_CAP > LOOPB_CAP, "cap order");
+    caps:
         if (__builtin_expect(u.cycles > LOOPB_CAP, 0)) {
             if (!u.in_a) return U_DONE;
             if (u.cycles > LOOPA_CAP) { st = ENV_VSYNC_HELD; return U_DONE; }
