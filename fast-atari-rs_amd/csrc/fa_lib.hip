@@ -258,6 +258,13 @@ int fa_set_sorted(fa_env *e, int sorted) {
 
 size_t fa_num_envs(const fa_env *e) { return e ? e->n : 0; }
 
+int fa_descriptor_table(uint32_t *out, size_t n_words) {
+    if (!out) return fail(FA_EINVAL, "out is NULL");
+    if (n_words != fa::kDescBytes / 4) return fail(FA_EINVAL, "n_words must be 1024");
+    fa::descriptor_table(out);
+    return FA_OK;
+}
+
 int fa_reset(fa_env *e, uint8_t *obs_out) {
     if (!e) return fail(FA_EINVAL, "NULL handle");
     int rc = fill_power_on(e);

@@ -141,6 +141,7 @@ def load_library():
         "fa_server_download": (C.c_int, [vp, C.c_int, vp, u64p, u64p, u32p]),
         "fa_server_client_info": (C.c_int, [vp, C.c_int, C.POINTER(sz), C.POINTER(sz)]),
         "fa_server_set_class_config": (C.c_int, [vp, sz, C.c_uint32, C.c_int]),
+        "fa_descriptor_table": (C.c_int, [u32p, sz]),
         "fa_vecenv_create": (C.c_int, [u8p, sz, sz, C.POINTER(vp)]),
         "fa_vecenv_destroy": (None, [vp]),
         "fa_vecenv_set_max_steps": (C.c_int, [vp, C.c_uint64]),

@@ -148,6 +148,10 @@ int fa_set_block_size(fa_env *env, uint32_t block);
 int fa_get_block_size(const fa_env *env, uint32_t *block);
 /* LDS bytes available to one workgroup on the handle's device. */
 int fa_get_max_lds(const fa_env *env, size_t *bytes);
+/* The per-opcode descriptor table the kernels stage into LDS: 256 opcodes x 4
+ * words (the decode d0 / d1 and the same decode as one-hot masks d2 / d3, see
+ * csrc/fa_decode.hpp); n_words must be 1024.  Host only (no GPU needed). */
+int fa_descriptor_table(uint32_t *out, size_t n_words);
 
 /* ---- heterogeneous batches (several ROM classes stepped together) ----
  * No reference counterpart: the reference binds one ROM per BatchAtariGpu
