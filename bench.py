@@ -216,7 +216,9 @@ def bench_serve(args, fastatari, W):
         "workload": f"256 clients, {n_envs} envs resident (64-2047 per client, log-uniform) on the 4 mixed ROMs; "
                     "Poisson arrivals at 0.5-2x saturation; 1% restarts per step; native load driver",
         "plain_batch_fps": round(plain_fps, 1),
-        "closed_loop": line(closed, np.zeros(len(specs))),
+        # run_load returns `steps` itself, which the open-loop phase advanced
+        # further: the closed phase's counts are the copy taken after it
+        "closed_loop": line(dict(closed, steps=closed_steps), np.zeros(len(specs))),
         "open_loop": line(opened, before.astype(np.float64)),
         "memory": mem,
     }

@@ -15,7 +15,8 @@ namespace fa {
 // HeadlessAtari::run_frame (src/atari.rs:584-614).  Returns the frame's
 // cycle count; sets `st` on an illegal opcode or a held VSYNC.
 //
-// Whenever the active lanes agree on the whole control state the wave runs the
+// Whenever the active lanes agree on the control state (the frame's cycle count
+// aside, see u_enter) the wave runs the
 // loop in the wave-uniform mode (fa_uniform.hpp) until the frame ends or an
 // instruction would make the lanes disagree; the per-lane loop below runs that
 // instruction and everything else.  Before each per-lane instruction a cheap
@@ -34,9 +35,8 @@ __device__ __forceinline__ uint32_t run_frame(Emu &e, uint32_t &st, uint32_t &cp
             if (u_maybe(e)) {
                 UCtl u;
                 if (u_enter(e, cycles, in_a, u)) {
-                    const uint32_t c0 = u.cycles;
                     const uint32_t r = u_run(u, e, st);
-                    const bool moved = u.cycles != c0;
+                    const bool moved = u.cycles != u.c0;
                     u_leave(u, e, cycles, in_a, cpu_cyc, instr);
                     if (r == U_DONE) break;
                     e.fetch_ahead(op, b1, b2);

@@ -1,8 +1,9 @@
 // fa_uniform.hpp -- the wave-uniform control mode of the frame loop.
 //
 // When every active lane of a wave holds the same control state -- PC, stack
-// pointer, bank, TIA line timing, VSYNC, the RIOT timer and the frame loop's
-// cycle count -- that state is kept once per wave in scalar registers and every
+// pointer, bank, TIA line timing, VSYNC and the RIOT timer (the frame loop's
+// cycle count may differ, see u_enter) -- that state is kept once per wave in
+// scalar registers and every
 // decision that depends on it (opcode dispatch, the bus region of an address,
 // TIA / RIOT side effects, line and frame ends) is a scalar branch.  Only what
 // a program computes on -- A, X, Y, P, the paddle counter and RAM -- stays per
@@ -30,11 +31,16 @@ struct UCtl {
     uint32_t clock, line, wsync, fcomp, vsync;   // wsync bit 1: see u_loop_bit
     uint32_t tval, tps, tsel, tsh, tuf;
     uint32_t tlim;           // timer fast-path budget (u_tlim)
-    uint32_t cycles, in_a;   // run_frame's cycle count and loop-A flag
+    uint32_t cycles, in_a;   // run_frame's cycle count (with cspread: a bound, see u_enter) and loop-A flag
+    uint32_t c0, cspread;    // cycles at entry; non-zero when the lanes' counts differ
     uint32_t cpu, instr;     // CPU cycles and instructions executed since entry
 };
 
 enum : uint32_t { U_DONE = 0, U_EXIT = 1 };
+
+// Cycles one iteration of the frame loop can add: a WSYNC skip (at most 76) or
+// one instruction (at most 7).
+constexpr uint32_t kIterCycles = 80u;
 
 // The first active lane's v, opaque to the optimiser: after a test that v
 // equals it on every lane it must not be replaced by the per-lane v.
@@ -63,7 +69,14 @@ __device__ __forceinline__ uint32_t u_loop_bit(const UCtl &u) {
 }
 
 // Full test; on success fills u from the common state.  Fields are compared at
-// the widths they have in the state image, so agreement is exact.
+// the widths they have in the state image, so agreement is exact.  The frame's
+// cycle count may differ between lanes by up to 127 (an env restarted from the
+// reset snapshot runs its first frame at another TIA phase, realigned by the
+// first WSYNC): it only feeds the frame-loop caps, so the mode then carries an
+// upper bound of every lane's count (the first lane's + 127), adds the cycles
+// it runs to each lane's own count on leaving, and leaves while the bound is
+// still at least one iteration (kIterCycles) short of the cap that applies, so
+// that the caps never separate the lanes inside the mode.
 __device__ __forceinline__ bool u_enter(const Emu &e, uint32_t cycles, bool in_a, UCtl &u) {
     const uint32_t w0 = e.pc | (e.sp << 16) | (e.bank << 24);
     const uint32_t w1 = e.clock | (e.line << 16);
@@ -72,8 +85,13 @@ __device__ __forceinline__ bool u_enter(const Emu &e, uint32_t cycles, bool in_a
     const uint32_t w4 = e.fcomp | (e.vsync << 8) | ((uint32_t)in_a << 16);
     const uint32_t s0 = wave_first(w0), s1 = wave_first(w1), s2 = wave_first(w2), s3 = wave_first(w3),
                    s4 = wave_first(w4), s5 = wave_first(cycles);
-    const bool diff = (w0 != s0) | (w1 != s1) | (w2 != s2) | (w3 != s3) | (w4 != s4) | (cycles != s5);
+    const bool diff = (w0 != s0) | (w1 != s1) | (w2 != s2) | (w3 != s3) | (w4 != s4) | (cycles - s5 + 127u > 254u);
     if (__ballot(diff) != 0) return false;
+    u.cspread = __ballot(cycles != s5) != 0 ? 1u : 0u;
+    u.cycles = u.cspread ? s5 + 127u : s5;
+    if (u.cspread && u.cycles > LOOPB_CAP - kIterCycles && (!(s4 & 0x10000u) || u.cycles > LOOPA_CAP - kIterCycles))
+        return false;
+    u.c0 = u.cycles;
     u.pc = s0 & 0xFFFFu;
     u.sp = (s0 >> 16) & 0xFFu;
     u.bank = s0 >> 24;
@@ -88,7 +106,6 @@ __device__ __forceinline__ bool u_enter(const Emu &e, uint32_t cycles, bool in_a
     u.fcomp = s4 & 0xFFu;
     u.vsync = (s4 >> 8) & 0xFFu;
     u.in_a = (s4 >> 16) & 1u;
-    u.cycles = s5;
     u.wsync |= u_loop_bit(u);
     u.cpu = 0;
     u.instr = 0;
@@ -100,7 +117,7 @@ __device__ __forceinline__ void u_leave(const UCtl &u, Emu &e, uint32_t &cycles,
     e.pc = u.pc; e.sp = u.sp; e.bank = u.bank;
     e.clock = u.clock; e.line = u.line; e.wsync = u.wsync & 1u; e.fcomp = u.fcomp; e.vsync = u.vsync;
     e.tval = u.tval; e.tps = u.tps; e.tsel = u.tsel; e.tsh = u.tsh; e.tuf = u.tuf;
-    cycles = u.cycles;
+    cycles += u.cycles - u.c0;
     in_a = u.in_a != 0u;
     cpu_cyc += u.cpu;
     instr += u.instr;
@@ -512,7 +529,8 @@ __device__ __forceinline__ uint32_t u_op(UCtl &u, Emu &e, uint32_t b1, uint32_t 
 // ---------------------------------------------------------------- frame loop
 // HeadlessAtari::run_frame's loop (src/atari.rs:584-614) on the uniform state.
 // Returns U_DONE when the frame ended (st set as the per-lane loop would), or
-// U_EXIT at the top of an iteration whose instruction needs the per-lane path.
+// U_EXIT at the top of an iteration whose instruction needs the per-lane path
+// (or, with the lanes' cycle counts spread, whose caps are near).
 __device__ __forceinline__ uint32_t u_run(UCtl &u, Emu &e, uint32_t &st) {
     // The optimiser may merge the entry values with per-lane values on the
     // path where the entry test failed; taking every field through an opaque
@@ -523,7 +541,7 @@ __device__ __forceinline__ uint32_t u_run(UCtl &u, Emu &e, uint32_t &st) {
     u.tval = wave_first(u.tval); u.tps = wave_first(u.tps); u.tsel = wave_first(u.tsel);
     u.tsh = wave_first(u.tsh); u.tuf = wave_first(u.tuf);
     u.tlim = u_tlim(u);
-    u.cycles = wave_first(u.cycles); u.in_a = wave_first(u.in_a);
+    u.cycles = wave_first(u.cycles); u.in_a = wave_first(u.in_a); u.cspread = wave_first(u.cspread);
     u.cpu = 0;
     u.instr = 0;
     // opcode and operand bytes at PC, loaded one iteration ahead so that the
@@ -541,7 +559,9 @@ __device__ __forceinline__ uint32_t u_run(UCtl &u, Emu &e, uint32_t &st) {
         if (__builtin_expect(u.wsync != 0u, 0)) {
             if (u.wsync & 2u) {
                 if (!u.in_a) return U_DONE;            // VSYNC rose in loop B: the frame ends
-                u.in_a = 0;                            // VSYNC dropped: loop A is over
+                // VSYNC dropped: loop A is over, and loop B's cap may be near
+                if (u.cspread && u.cycles > LOOPB_CAP - kIterCycles) return U_EXIT;
+                u.in_a = 0;
                 u.wsync &= 1u;
             }
             if (u.wsync) {
@@ -587,9 +607,14 @@ __device__ __forceinline__ uint32_t u_run(UCtl &u, Emu &e, uint32_t &st) {
         // the frame-loop caps (loop A's is the larger)
         static_assert(LOOPA_CAP > LOOPB_CAP, "cap order");
     caps:
-        if (__builtin_expect(u.cycles > LOOPB_CAP, 0)) {
-            if (!u.in_a) return U_DONE;
-            if (u.cycles > LOOPA_CAP) { st = ENV_VSYNC_HELD; return U_DONE; }
+        if (__builtin_expect(u.cycles > LOOPB_CAP - kIterCycles, 0)) {
+            if (u.cspread) {
+                // no lane has passed its cap yet; the per-lane loop goes on
+                if (!u.in_a || u.cycles > LOOPA_CAP - kIterCycles) return U_EXIT;
+            } else if (u.cycles > LOOPB_CAP) {
+                if (!u.in_a) return U_DONE;
+                if (u.cycles > LOOPA_CAP) { st = ENV_VSYNC_HELD; return U_DONE; }
+            }
         }
     }
 }

@@ -229,6 +229,34 @@ def test_frame_cap_without_vsync(fa, orc):
     assert int(gpu.counters()["frame_cycles"][0]) > 100000
 
 
+@pytest.mark.parametrize("sorted_", [False, True])
+def test_frame_caps_with_staggered_lanes(fa, orc, sorted_):
+    """Lanes that agree on everything but the frame's cycle count (TIA clocks
+    staggered, realigned by the first WSYNC) run the wave-uniform mode
+    together; the caps (src/atari.rs:592-595: loop B at 100,000 cycles, and the
+    held-VSYNC cap of loop A) still end each lane's frame at its own count."""
+    rom = _rom_with([0x85, 0x02, 0xE6, 0x80, 0x4C, 0x00, 0xF0])   # STA WSYNC; INC $80; JMP $F000
+    n = 128
+    gpu, ref = _pair(fa, orc, rom, n, sorted_=sorted_)
+    s = ref.states().copy()
+    for i in range(n):
+        s[i, 152:154] = np.frombuffer(np.uint16(3 * ((i * 7) % 76)).tobytes(), np.uint8)   # TIA clock
+        if i >= 64:
+            s[i, 158] = 2                                            # VSYNC held: loop A
+    ref.set_states(s)
+    gpu.upload_states(s)
+    spread = []
+    for f in range(3):
+        a = np.zeros(n, np.uint8)
+        gpu.step(a)
+        ref.run(a[None, :])
+        assert_same(gpu, ref, f"frame {f}")
+        spread.append(len(set(ref.counters()[:64, 1].tolist())))
+    assert spread[0] > 1                                             # the lanes' frame counts differed
+    c = ref.counters()
+    assert (c[64:, 3] == 2).all()                                    # loop A hit its cap: VSYNC held
+
+
 def test_hotspot_mid_instruction_switch(fa, orc):
     """Fetches through the F8 hotspots switch banks mid-instruction
     (src/atari.rs:504-507): the phantom read of NOP at $FFF7 selects bank 0,
