@@ -13,7 +13,9 @@ one autotuned hipGraph launch of K = 8 steps with per-env frameskip) and the
 served mode ("served": 256 clients with continuous batching; p50 / p99 step
 latency beside aggregate frames/s; its bit-exactness, 1 % of envs against the
 oracle, is checked in tests/test_gpu_serve.py) and the env adapter
-("env_adapter": BatchAtariGpuEnv steps with f32 observations).
+("env_adapter": BatchAtariGpuEnv steps with f32 observations).  "state_io"
+is the HBM rate of the frame-boundary state traffic alone (launches that run
+zero frames per env: state and counters in, state, counters and obs out).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -314,6 +316,24 @@ def main():
     api_fps = n * api_frames / (time.perf_counter() - t0)
     gpu.kernel_time(reset=True)
 
+    # ---- the frame-boundary state traffic alone: zero frames per env, so a
+    # launch stages state and counters in and writes state, counters and obs
+    # back without emulating
+    d_fs0 = fastatari.DeviceBuffer(n)
+    d_fs0.upload(np.zeros(n, np.uint8))
+    for _ in range(4):
+        gpu.step_frameskip_device(d_act.ptr, d_fs0.ptr, d_obs.ptr)
+    gpu.sync()
+    gpu.kernel_time(reset=True)
+    io_launches = 32
+    for _ in range(io_launches):
+        gpu.step_frameskip_device(d_act.ptr, d_fs0.ptr, d_obs.ptr)
+    gpu.sync()
+    io_ms_total, io_n = gpu.kernel_time(reset=True)
+    io_ms = io_ms_total / max(io_n, 1)
+    io_gbs = (BYTES_PER_ENV_FRAME + 1) * n / (io_ms * 1e-3) / 1e9     # + the frameskip byte
+    d_fs0.free()
+
     achieved_gbs = BYTES_PER_ENV_FRAME * n / (kernel_ms * 1e-3) / 1e9
     measured = {}
     if os.path.exists(PROFILE_FIGURES):
@@ -346,6 +366,10 @@ def main():
         "env_errors": status_bad,
         "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved_gbs / HBM_PEAK_GBS, 6), "traffic": measured.get("traffic_bytes")},
+        "state_io": {"achieved": round(io_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(io_gbs / HBM_PEAK_GBS, 4), "ms_per_launch": round(io_ms, 4),
+                     "bytes_per_launch": (BYTES_PER_ENV_FRAME + 1) * n,
+                     "note": "zero-frame launches: state + counters in, state + counters + obs out"},
     }
     if "valu_wave_instr" in measured:
         valu = measured["valu_wave_instr"] / (kernel_ms * 1e-3) / 1e9
