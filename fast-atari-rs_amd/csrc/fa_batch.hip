@@ -7,8 +7,8 @@
 // indexed by the caller's global env index, one ROM image per class.  A step
 // of K frames-held-for-fs is a graph of per-class kernel chains that fork from
 // the batch stream, run concurrently on per-class streams and join back;
-// each class has its own launch shape (block size, opcode grouping, steps per
-// launch), chosen by fa_batch_tune from timed trials.
+// each class has its own launch shape (block size, envs per workgroup, opcode
+// grouping, steps per launch), chosen by fa_batch_tune from timed trials.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,9 +32,18 @@ struct BatchClass {
     hipEvent_t join = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     uint32_t block = 0;            // 0: largest that fits the LDS
+    uint32_t lanes = 0;            // envs per workgroup below the block (8, 16, 32), 0: the block
     int sorted = 0;
     uint32_t k_split = 0;          // steps per kernel launch, 0: all
+    uint32_t *d_tasks[3] = {};     // (first, count) per workgroup for 32, 16 and 8 envs a workgroup
 };
+
+// Fewer envs per workgroup than its lanes spread a class over more waves:
+// a divergent wave's frame takes longer the more lanes (and so the more
+// different instruction kinds) it holds, so a class that leaves SIMDs idle
+// can trade lanes for waves.
+bool lanes_ok(uint32_t lanes) { return lanes == 0 || lanes == 8 || lanes == 16 || lanes == 32; }
+int lanes_index(uint32_t lanes) { return lanes == 32 ? 0 : (lanes == 16 ? 1 : 2); }
 
 constexpr size_t kSlotAlign = 256;
 
@@ -87,16 +96,19 @@ fa::StepArgs class_args(const fa_batch *b, const BatchClass &c) {
 
 // The kernel chain of one class for k steps, on stream s.
 int enqueue_class(fa_batch *b, const BatchClass &c, uint32_t k, bool use_fs, bool use_actions, int obs_mode,
-                  uint32_t block, int sorted, uint32_t k_split, hipStream_t s) {
+                  uint32_t block, uint32_t lanes, int sorted, uint32_t k_split, hipStream_t s) {
     if (c.count == 0) return FA_OK;
     const uint32_t bs = block;
     if (fa::lds_bytes(c.rom.rom_bytes, bs) > b->max_lds) return fail(FA_EINVAL, "block too large for the LDS budget");
-    const uint32_t grid = (uint32_t)((c.count + bs - 1) / bs);
+    const bool spread = lanes != 0 && lanes < bs;
+    const uint32_t per = spread ? lanes : bs;
+    const uint32_t grid = (uint32_t)((c.count + per - 1) / per);
     const uint32_t ks = k_split ? std::min(k_split, k) : k;
     for (uint32_t s0 = 0; s0 < k; s0 += ks) {
         fa::StepArgs g = class_args(b, c);
         const size_t step0 = (size_t)s0 * b->n;
         g.k = std::min(ks, k - s0);
+        if (spread) g.tasks = c.d_tasks[lanes_index(lanes)];
         g.actions = use_actions ? b->d_actions + step0 + c.first : nullptr;
         g.fs = use_fs ? b->d_fs + step0 + c.first : nullptr;
         if (obs_mode == 1)
@@ -116,7 +128,8 @@ int enqueue_all(fa_batch *b, uint32_t k, bool use_fs, bool use_actions, int obs_
     for (auto &c : b->cls) {
         if (c.count == 0) continue;
         FA_HIP_TRY(hipStreamWaitEvent(c.stream, b->fork, 0));
-        int rc = enqueue_class(b, c, k, use_fs, use_actions, obs_mode, class_block(b, c), c.sorted, c.k_split, c.stream);
+        int rc = enqueue_class(b, c, k, use_fs, use_actions, obs_mode, class_block(b, c), c.lanes, c.sorted, c.k_split,
+                               c.stream);
         if (rc) return rc;
         FA_HIP_TRY(hipEventRecord(c.join, c.stream));
         FA_HIP_TRY(hipStreamWaitEvent(b->stream, c.join, 0));
@@ -232,6 +245,18 @@ int fa_batch_create(const uint8_t *const *roms, const size_t *rom_lens, const si
         if (hipMalloc(&c.d_rom, c.rom.rom_bytes) != hipSuccess) return cleanup(fail(FA_ENOMEM, "ROM allocation failed"));
         if (hipMemcpy(c.d_rom, c.rom.image.data(), c.rom.rom_bytes, hipMemcpyHostToDevice) != hipSuccess)
             return cleanup(fail(FA_EHIP, "ROM upload failed"));
+        // task lists for the spread launch shapes (env ranges start at multiples of 8)
+        for (int li = 0; li < 3 && c.count; ++li) {
+            const uint32_t per = 32u >> li;
+            std::vector<uint32_t> t;
+            for (size_t off = 0; off < c.count; off += per) {
+                t.push_back((uint32_t)off);
+                t.push_back((uint32_t)std::min<size_t>(per, c.count - off));
+            }
+            if (hipMalloc(&c.d_tasks[li], t.size() * 4) != hipSuccess) return cleanup(fail(FA_ENOMEM, "task list allocation failed"));
+            if (hipMemcpy(c.d_tasks[li], t.data(), t.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+                return cleanup(fail(FA_EHIP, "task list upload failed"));
+        }
     }
     if (ensure_inputs(b, 1, 1)) return cleanup(FA_ENOMEM);
     // power-on state until reset
@@ -254,6 +279,8 @@ void fa_batch_destroy(fa_batch *b) {
     for (auto &c : b->cls) {
         if (c.stream) hipStreamSynchronize(c.stream);
         if (c.d_rom) hipFree(c.d_rom);
+        for (uint32_t *t : c.d_tasks)
+            if (t) hipFree(t);
         if (c.join) hipEventDestroy(c.join);
         if (c.t0) hipEventDestroy(c.t0);
         if (c.t1) hipEventDestroy(c.t1);
@@ -407,6 +434,24 @@ int fa_batch_set_class_config(fa_batch *b, size_t c, uint32_t block, int sorted,
     return FA_OK;
 }
 
+int fa_batch_set_class_lanes(fa_batch *b, size_t c, uint32_t lanes) {
+    if (check_batch(b)) return FA_EINVAL;
+    if (c >= b->cls.size()) return fail(FA_EINVAL, "class index out of range");
+    if (!lanes_ok(lanes)) return fail(FA_EINVAL, "lanes must be 0, 8, 16 or 32");
+    FA_HIP_TRY(hipStreamSynchronize(b->stream));
+    b->cls[c].lanes = lanes;
+    drop_graphs(b);
+    return FA_OK;
+}
+
+int fa_batch_get_class_lanes(const fa_batch *b, size_t c, uint32_t *lanes) {
+    if (check_batch(b)) return FA_EINVAL;
+    if (c >= b->cls.size()) return fail(FA_EINVAL, "class index out of range");
+    if (!lanes) return fail(FA_EINVAL, "lanes is NULL");
+    *lanes = b->cls[c].lanes;
+    return FA_OK;
+}
+
 int fa_batch_get_class_config(const fa_batch *b, size_t c, uint32_t *block, int *sorted, uint32_t *k_split) {
     if (check_batch(b)) return FA_EINVAL;
     if (c >= b->cls.size()) return fail(FA_EINVAL, "class index out of range");
@@ -419,8 +464,10 @@ int fa_batch_get_class_config(const fa_batch *b, size_t c, uint32_t *block, int 
 // Autotuner: for each class, times trial launches of that class alone on the
 // inputs currently in the batch's buffers (k steps; frame counts from the fs
 // buffer when use_fs), restoring the class's states and counters after every
-// trial.  Stage 1 picks opcode grouping and block size on a one-step trial;
-// stage 2 picks steps per launch (a divisor chain of k) on the full k steps.
+// trial.  Stage 1 picks opcode grouping and block size on a one-step trial,
+// and for 64-env workgroups also 32, 16 or 8 envs per workgroup (more, less
+// full waves); stage 2 picks steps per launch (a divisor chain of k) on the
+// full k steps.
 // The winners become the class configs; trial times land in `trials`
 // (per class: [ms of the chosen config, ms of the default config]).
 int fa_batch_tune(fa_batch *b, uint32_t k, int use_fs, double *trials) {
@@ -462,11 +509,11 @@ int fa_batch_tune(fa_batch *b, uint32_t k, int use_fs, double *trials) {
             }
             return FA_OK;
         };
-        auto trial = [&](uint32_t steps, uint32_t block, int sorted, uint32_t ks, double *ms) -> int {
+        auto trial = [&](uint32_t steps, uint32_t block, uint32_t lanes, int sorted, uint32_t ks, double *ms) -> int {
             int r = copy_all(false);
             if (r) return r;
             FA_HIP_TRY(hipEventRecord(c.t0, c.stream));
-            r = enqueue_class(b, c, steps, use_fs != 0, true, 0, block, sorted, ks, c.stream);
+            r = enqueue_class(b, c, steps, use_fs != 0, true, 0, block, lanes, sorted, ks, c.stream);
             if (r) return r;
             FA_HIP_TRY(hipEventRecord(c.t1, c.stream));
             FA_HIP_TRY(hipStreamSynchronize(c.stream));
@@ -478,29 +525,33 @@ int fa_batch_tune(fa_batch *b, uint32_t k, int use_fs, double *trials) {
         rc = copy_all(true);
         if (rc) return rc;
         // stage 1: block x grouping on one step
-        uint32_t best_block = fa::default_block(c.rom.rom_bytes, b->max_lds);
+        uint32_t best_block = fa::default_block(c.rom.rom_bytes, b->max_lds), best_lanes = 0;
         int best_sorted = 0;
         double best = 1e30;
         for (int sorted = 0; sorted < 2; ++sorted)
             for (uint32_t block : {64u, 128u, 256u}) {
                 if (fa::lds_bytes(c.rom.rom_bytes, block) > b->max_lds) continue;
-                double ms = 0;
-                rc = trial(1, block, sorted, 0, &ms);
-                if (rc) return rc;
-                if (ms < best) {
-                    best = ms;
-                    best_block = block;
-                    best_sorted = sorted;
+                for (uint32_t lanes : {0u, 32u, 16u, 8u}) {
+                    if (lanes && block != 64u) break;
+                    double ms = 0;
+                    rc = trial(1, block, lanes, sorted, 0, &ms);
+                    if (rc) return rc;
+                    if (ms < best) {
+                        best = ms;
+                        best_block = block;
+                        best_lanes = lanes;
+                        best_sorted = sorted;
+                    }
                 }
             }
         // stage 2: steps per launch on the full k
         double def_ms = 0, best_k = 1e30;
         uint32_t best_split = 0;
-        rc = trial(k, fa::default_block(c.rom.rom_bytes, b->max_lds), 0, 0, &def_ms);
+        rc = trial(k, fa::default_block(c.rom.rom_bytes, b->max_lds), 0, 0, 0, &def_ms);
         if (rc) return rc;
         for (uint32_t ks = k;; ks = (ks + 1) / 2) {
             double ms = 0;
-            rc = trial(k, best_block, best_sorted, ks, &ms);
+            rc = trial(k, best_block, best_lanes, best_sorted, ks, &ms);
             if (rc) return rc;
             if (ms < best_k) {
                 best_k = ms;
@@ -512,6 +563,7 @@ int fa_batch_tune(fa_batch *b, uint32_t k, int use_fs, double *trials) {
         if (rc) return rc;
         FA_HIP_TRY(hipStreamSynchronize(c.stream));
         c.block = best_block;
+        c.lanes = best_lanes;
         c.sorted = best_sorted;
         c.k_split = best_split == k ? 0 : best_split;
         if (trials) {

@@ -141,10 +141,15 @@ class BatchMixGpu:
         _check(load_library().fa_batch_set_class_config(self._h, int(c), int(block), int(bool(sorted_)),
                                                         int(k_split)))
 
+    def set_class_lanes(self, c: int, lanes: int):
+        """Envs per workgroup below the block (8, 16, 32; 0: the block)."""
+        _check(load_library().fa_batch_set_class_lanes(self._h, int(c), int(lanes)))
+
     def class_config(self, c: int) -> dict:
-        b, s, k = C.c_uint32(), C.c_int(), C.c_uint32()
+        b, s, k, ln = C.c_uint32(), C.c_int(), C.c_uint32(), C.c_uint32()
         _check(load_library().fa_batch_get_class_config(self._h, int(c), C.byref(b), C.byref(s), C.byref(k)))
-        return {"block": b.value, "sorted": bool(s.value), "k_split": k.value}
+        _check(load_library().fa_batch_get_class_lanes(self._h, int(c), C.byref(ln)))
+        return {"block": b.value, "lanes": ln.value, "sorted": bool(s.value), "k_split": k.value}
 
     def tune(self, actions, fs=None) -> list[dict]:
         """Autotunes every class on these inputs ([k, N]); env states are

@@ -129,6 +129,8 @@ def load_library():
         "fa_batch_download_counters": (C.c_int, [vp, u64p, u64p, u64p, u32p]),
         "fa_batch_set_class_config": (C.c_int, [vp, sz, C.c_uint32, C.c_int, C.c_uint32]),
         "fa_batch_get_class_config": (C.c_int, [vp, sz, u32p, C.POINTER(C.c_int), u32p]),
+        "fa_batch_set_class_lanes": (C.c_int, [vp, sz, C.c_uint32]),
+        "fa_batch_get_class_lanes": (C.c_int, [vp, sz, u32p]),
         "fa_batch_tune": (C.c_int, [vp, C.c_uint32, C.c_int, C.POINTER(C.c_double)]),
         # server
         "fa_server_create": (C.c_int, [C.POINTER(u8p), C.POINTER(sz), sz, sz, C.POINTER(vp)]),

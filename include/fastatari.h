@@ -189,9 +189,15 @@ int fa_batch_download_counters(fa_batch *b, uint64_t *cpu_cycles, uint64_t *fram
  * steps per kernel launch (0 = all k in one launch). */
 int fa_batch_set_class_config(fa_batch *b, size_t c, uint32_t block, int sorted, uint32_t k_split);
 int fa_batch_get_class_config(const fa_batch *b, size_t c, uint32_t *block, int *sorted, uint32_t *k_split);
+/* Envs per workgroup below the block: 8, 16 or 32 spread the class over more,
+ * less full waves (a divergent wave's frame is shorter with fewer lanes); 0 =
+ * the block.  No counterpart in the reference, whose kernels run one fixed
+ * launch shape (cuda/src/atari_kernel.cu:104-113). */
+int fa_batch_set_class_lanes(fa_batch *b, size_t c, uint32_t lanes);
+int fa_batch_get_class_lanes(const fa_batch *b, size_t c, uint32_t *lanes);
 /* Autotuner: times trial launches of each class alone on the inputs in the
  * batch buffers (states restored after every trial) and adopts the fastest
- * block / grouping / steps-per-launch.  trials (2 * nclass, may be NULL):
+ * block / envs per workgroup / grouping / steps-per-launch.  trials (2 * nclass, may be NULL):
  * per class, ms of the chosen and of the default configuration. */
 int fa_batch_tune(fa_batch *b, uint32_t k, int use_fs, double *trials);
 

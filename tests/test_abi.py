@@ -52,3 +52,17 @@ def test_null_handles_are_errors_not_crashes():
     L.fa_batch_destroy(None)
     L.fa_server_destroy(None)
     L.fa_vecenv_destroy(None)
+
+
+def test_rust_build_script_compiles_what_the_python_build_compiles():
+    """rust/build.rs (no Rust toolchain here) names the same translation units
+    with the same extra options as fastatari/build.py."""
+    from fastatari import build
+    text = open(os.path.join(ROOT, "fast-atari-rs_amd", "rust", "build.rs")).read()
+    pairs = re.findall(r'\("(fa_[a-z_]+\.hip)",\s*(uniform_skip|&\[\])\)', text)
+    rust = {src: (build.UNIFORM_SKIP if opt == "uniform_skip" else []) for src, opt in pairs}
+    assert rust == {src: list(extra) for src, extra in build.SOURCES}
+    m = re.search(r"let uniform_skip: &\[&str\] = &\[(.*?)\];", text)
+    assert m and re.findall(r'"([^"]+)"', m.group(1)) == build.UNIFORM_SKIP
+    flags = re.findall(r'"([^"]+)"', re.search(r"let flags = \[(.*?)\];", text, re.S).group(1))
+    assert flags == [f for f in build.FLAGS if not f.startswith("-Wno-")]

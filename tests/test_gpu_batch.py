@@ -117,7 +117,7 @@ def test_launch_shapes_agree(fa_mod, orc, W, roms, block, sorted_, k_split):
     for c in range(4):
         mix.set_class_config(c, block, sorted_, k_split)
         cfg = mix.class_config(c)
-        assert cfg == {"block": block, "sorted": sorted_, "k_split": k_split}
+        assert cfg == {"block": block, "lanes": 0, "sorted": sorted_, "k_split": k_split}
     refs = _refs(orc, roms, counts)
     ranges = [mix.class_range(c) for c in range(4)]
     mix.reset()
@@ -125,6 +125,39 @@ def test_launch_shapes_agree(fa_mod, orc, W, roms, block, sorted_, k_split):
     mix.step(acts, fs)
     _run_refs(refs, ranges, acts, fs)
     _check(mix, refs, ranges, f"block {block} sorted {sorted_} k_split {k_split}")
+
+
+@pytest.mark.parametrize("block,lanes,sorted_,k_split", [(64, 8, False, 0), (64, 16, True, 3), (64, 32, False, 1),
+                                                         (128, 16, False, 0)])
+def test_spread_launch_shapes_agree(fa_mod, orc, W, roms, block, lanes, sorted_, k_split):
+    """Fewer envs per workgroup than its lanes (task lists of 8 / 16 / 32
+    envs, the last one partial), also with opcode grouping, split launches,
+    the graph path and obs of every step."""
+    counts = [100, 200, 90, 70]
+    mix = fa_mod.BatchMixGpu(roms, counts)
+    for c in range(4):
+        mix.set_class_config(c, block, sorted_, k_split)
+        mix.set_class_lanes(c, lanes)
+        assert mix.class_config(c) == {"block": block, "lanes": lanes, "sorted": sorted_, "k_split": k_split}
+    refs = _refs(orc, roms, counts)
+    ranges = [mix.class_range(c) for c in range(4)]
+    mix.reset()
+    acts, fs = _inputs(W, mix.n, 8, 40, "per_step")
+    obs = mix.step(acts, fs, obs_every=True)
+    for s in range(8):
+        for (first, cnt), ref in zip(ranges, refs):
+            ref.run_frameskip(acts[s, first:first + cnt], fs[s, first:first + cnt])
+            assert np.array_equal(obs[s, first:first + cnt], ref.obs()), f"obs step {s}"
+    _check(mix, refs, ranges, f"lanes {lanes} block {block}")
+    for launch in range(2):
+        acts, fs = _inputs(W, mix.n, 8, 41 + launch, "per_step")
+        mix.write_inputs(acts, fs)
+        mix.launch(8, use_fs=True)
+        mix.sync()
+        _run_refs(refs, ranges, acts, fs)
+        _check(mix, refs, ranges, f"lanes {lanes} graph replay {launch}")
+    with pytest.raises(ValueError):
+        mix.set_class_lanes(0, 24)
 
 
 def test_tune_keeps_state_and_parity(fa_mod, orc, W, roms):
@@ -143,6 +176,7 @@ def test_tune_keeps_state_and_parity(fa_mod, orc, W, roms):
     assert len(res) == 4
     for r in res:
         assert r["block"] in (64, 128, 256) and r["tuned_ms"] > 0 and r["default_ms"] > 0
+        assert r["lanes"] in (0, 8, 16, 32) and (r["lanes"] == 0 or r["block"] == 64)
     assert np.array_equal(mix.download_states_bytes(), before), "tuning changed env state"
     cnt_after = mix.counters()
     for k in cnt_before:
