@@ -96,6 +96,7 @@ struct LaunchCtx {
     size_t task_cap = 0;
     std::vector<int> clients;
     bool busy = false;
+    Clock::time_point t_launch{};      // when the launch was issued
 };
 
 struct ServerClass {
@@ -229,10 +230,10 @@ void finish_launch(LaunchCtx *ctx);
 
 // Dispatcher: one launch per iteration; waits for queued work and a free
 // launch context of the same class.  After a class's launch completes, its
-// clients resubmit one by one: the class is held for up to coalesce_us (or
-// until every open client of the class has a request queued) so that the
-// next launch takes them together instead of running a near-empty launch
-// first.
+// clients resubmit one by one: the class is held for up to coalesce_us or a
+// tenth of its last launch's time, whichever is longer (or until every open
+// client of the class has a request queued) so that the next launch takes
+// them together instead of running a near-empty launch first.
 void dispatch_loop(fa_server *s) {
     std::unique_lock<std::mutex> lk(s->mu);
     size_t rr = 0;
@@ -330,6 +331,7 @@ void dispatch_loop(fa_server *s) {
             g.packed = nullptr;
             g.obs = nullptr;
             g.htasks = ctx->dv_tasks;
+            ctx->t_launch = Clock::now();
             hip_ok(fa::launch_step(g, sorted != 0, bs, t, ctx->stream), "frame kernel");
             hip_ok(hipEventRecord(ctx->done, ctx->stream), "hipEventRecord");
         }
@@ -386,7 +388,12 @@ void finish_launch(LaunchCtx *ctx) {
             if (s->track_completions) s->completions.push_back(id);
         }
         ctx->busy = false;
-        s->cls[ctx->cls].hold_until = now + std::chrono::microseconds(s->coalesce_us);
+        // the window grows with the class's launch time: the clients of a
+        // class whose frames take tens of ms resubmit over a few ms, and a
+        // launch that leaves some behind costs them a whole further launch
+        const auto hold = std::max<Clock::duration>(std::chrono::microseconds(s->coalesce_us),
+                                                    (now - ctx->t_launch) / 10);
+        s->cls[ctx->cls].hold_until = now + hold;
     }
     s->cv_done.notify_all();
     s->cv_dispatch.notify_all();

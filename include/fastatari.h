@@ -235,8 +235,9 @@ int fa_server_client_info(fa_server *s, int client_id, size_t *cls, size_t *n_en
  * another class's workgroup) and opcode grouping. */
 int fa_server_set_class_config(fa_server *s, size_t cls, uint32_t block, int sorted);
 /* Coalescing window (default 300 us): after a launch of a class completes,
- * its next launch waits up to this long for the class's clients to resubmit
- * (it starts at once when all of them have). */
+ * its next launch waits up to this long, or a tenth of the completed launch's
+ * time if that is longer, for the class's clients to resubmit (it starts at
+ * once when all of them have). */
 int fa_server_set_coalesce_us(fa_server *s, uint32_t us);
 int fa_server_memory(fa_server *s, size_t *used_bytes, size_t *budget_bytes, size_t *bytes_per_slot);
 /* Per completed request since the last reset: latency ms, class, envs,
