@@ -83,7 +83,8 @@ hipError_t launch_fill(const AtariStateGpu &tmpl, uint8_t *states, uint64_t *cpu
                        uint32_t *status, uint32_t n, uint32_t n_pad, hipStream_t s);
 void descriptor_table(uint32_t out[kDescBytes / 4]);
 
-// fa_lib.hip: the batch handle's stream and device obs buffer (n * 128 B)
+// fa_lib.hip: the batch handle's stream and device obs buffer (n * 128 B; it
+// holds the obs of device-side steps and of resets that return no host obs)
 hipStream_t env_stream(fa_env *e);
 uint8_t *env_obs_device(fa_env *e);
 

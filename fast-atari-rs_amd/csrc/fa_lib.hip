@@ -3,9 +3,11 @@
 // and error reporting shared with the batch and server translation units.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "fa_internal.hpp"
@@ -113,6 +115,7 @@ struct fa_env {
     uint64_t *d_cpu = nullptr, *d_fcyc = nullptr, *d_instr = nullptr;
     uint32_t *d_status = nullptr;
     size_t actions_cap = 0;
+    uint8_t *h_obs = nullptr, *hv_obs = nullptr;   // pinned obs for host-side steps, and its device address
 
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -135,6 +138,37 @@ int ensure_actions(fa_env *e, size_t bytes) {
     FA_HIP_TRY(hipMalloc(&e->d_actions, cap));
     e->actions_cap = cap;
     return FA_OK;
+}
+
+// Host-side steps: the kernel writes the obs straight into pinned host memory
+// (no separate device-to-host copy), and a few threads copy them into the
+// caller's buffer (one core copies 8 MB at a fraction of the host's memory
+// bandwidth).
+int host_obs(fa_env *e, uint8_t **dev) {
+    if (!e->h_obs) {
+        FA_HIP_TRY(hipHostMalloc(&e->h_obs, std::max<size_t>(e->n, 1) * 128, hipHostMallocDefault));
+        FA_HIP_TRY(hipHostGetDevicePointer((void **)&e->hv_obs, e->h_obs, 0));
+    }
+    *dev = e->hv_obs;
+    return FA_OK;
+}
+
+void copy_obs(uint8_t *dst, const uint8_t *src, size_t bytes) {
+    constexpr size_t kPart = 1u << 20;
+    const size_t parts = std::min<size_t>(8, (bytes + kPart - 1) / kPart);
+    if (parts <= 1) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t per = (bytes + parts - 1) / parts;
+    std::vector<std::thread> ts;
+    for (size_t t = 1; t < parts; ++t) {
+        const size_t off = t * per;
+        if (off >= bytes) break;
+        ts.emplace_back([=] { std::memcpy(dst + off, src + off, std::min(per, bytes - off)); });
+    }
+    std::memcpy(dst, src, std::min(per, bytes));
+    for (auto &t : ts) t.join();
 }
 
 // k steps; each holds its action for fs[s * n + i] frames (1 without fs).
@@ -246,6 +280,7 @@ void fa_destroy(fa_env *e) {
     for (void *p : {(void *)e->d_desc, (void *)e->d_rom, (void *)e->d_states, (void *)e->d_actions, (void *)e->d_fs,
                     (void *)e->d_obs, (void *)e->d_cpu, (void *)e->d_fcyc, (void *)e->d_instr, (void *)e->d_status})
         if (p) hipFree(p);
+    if (e->h_obs) hipHostFree(e->h_obs);
     if (e->stream) hipStreamDestroy(e->stream);
     delete e;
 }
@@ -269,10 +304,12 @@ int fa_reset(fa_env *e, uint8_t *obs_out) {
     if (!e) return fail(FA_EINVAL, "NULL handle");
     int rc = fill_power_on(e);
     if (rc) return rc;
-    rc = launch_frames(e, nullptr, nullptr, 1, e->d_obs);
+    uint8_t *obs = e->d_obs;
+    if (obs_out && (rc = host_obs(e, &obs))) return rc;
+    rc = launch_frames(e, nullptr, nullptr, 1, obs);
     if (rc) return rc;
-    if (obs_out) FA_HIP_TRY(hipMemcpyAsync(obs_out, e->d_obs, e->n * 128, hipMemcpyDeviceToHost, e->stream));
     FA_HIP_TRY(hipStreamSynchronize(e->stream));
+    if (obs_out) copy_obs(obs_out, e->h_obs, e->n * 128);
     return FA_OK;
 }
 
@@ -284,10 +321,12 @@ int fa_step_multi(fa_env *e, const uint8_t *actions, size_t n_actions, uint32_t 
     int rc = ensure_actions(e, n_actions);
     if (rc) return rc;
     if (n_actions) FA_HIP_TRY(hipMemcpyAsync(e->d_actions, actions, n_actions, hipMemcpyHostToDevice, e->stream));
-    rc = launch_frames(e, e->d_actions, nullptr, k, e->d_obs);
+    uint8_t *obs = e->d_obs;
+    if (obs_out && (rc = host_obs(e, &obs))) return rc;
+    rc = launch_frames(e, e->d_actions, nullptr, k, obs);
     if (rc) return rc;
-    if (obs_out) FA_HIP_TRY(hipMemcpyAsync(obs_out, e->d_obs, e->n * 128, hipMemcpyDeviceToHost, e->stream));
     FA_HIP_TRY(hipStreamSynchronize(e->stream));
+    if (obs_out) copy_obs(obs_out, e->h_obs, e->n * 128);
     return FA_OK;
 }
 
@@ -305,10 +344,12 @@ int fa_step_frameskip(fa_env *e, const uint8_t *actions, const uint8_t *fs, size
         FA_HIP_TRY(hipMemcpyAsync(e->d_actions, actions, n, hipMemcpyHostToDevice, e->stream));
         FA_HIP_TRY(hipMemcpyAsync(e->d_fs, fs, n, hipMemcpyHostToDevice, e->stream));
     }
-    rc = launch_frames(e, e->d_actions, e->d_fs, 1, e->d_obs);
+    uint8_t *obs = e->d_obs;
+    if (obs_out && (rc = host_obs(e, &obs))) return rc;
+    rc = launch_frames(e, e->d_actions, e->d_fs, 1, obs);
     if (rc) return rc;
-    if (obs_out) FA_HIP_TRY(hipMemcpyAsync(obs_out, e->d_obs, e->n * 128, hipMemcpyDeviceToHost, e->stream));
     FA_HIP_TRY(hipStreamSynchronize(e->stream));
+    if (obs_out) copy_obs(obs_out, e->h_obs, e->n * 128);
     return FA_OK;
 }
 
