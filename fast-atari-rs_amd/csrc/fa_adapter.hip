@@ -7,7 +7,8 @@
 // repeated (src/rl4burn_env.rs:339-353: set_action writes only input
 // registers the ROM cannot change, so setting it once or every frame is the
 // same).  The last frame's RAM becomes f32 observations on the device
-// (byte / 255, src/rl4burn_env.rs:314-316) before one copy to the host.  An
+// (byte / 255, src/rl4burn_env.rs:314-316), written by the kernel straight
+// into pinned host memory and copied from there into the caller's buffer.  An
 // optional RAM reader turns each env's RAM into a reward (score change) and a
 // terminated flag (lives field 0), also on the device; without one, reward
 // and terminated are the reference's constants 0 and false.
@@ -27,7 +28,8 @@ struct fa_vecenv {
     uint8_t *d_actions = nullptr;      // n: the step's actions
     uint8_t *d_fs = nullptr;           // n: frameskip of every env
     uint32_t fs_held = 0;              // the value d_fs holds (0: not written yet)
-    float *d_obs = nullptr;            // n * 128 f32 observations for host copies
+    float *h_obs = nullptr;            // n * 128 f32 observations for host copies, in pinned host memory
+    float *hv_obs = nullptr;           // its device address, which the f32 kernel writes
     // RAM reader (reward / termination)
     bool reading = false;
     fa_ram_reader reader{};
@@ -119,13 +121,12 @@ void count_step(fa_vecenv *v, float *reward, int *terminated, int *truncated) {
 
 int finish_host(fa_vecenv *v, float *obs_out) {
     hipStream_t s = fa::env_stream(v->env);
-    if (obs_out)
-        FA_HIP_TRY(hipMemcpyAsync(obs_out, v->d_obs, v->n * FA_OBS_BYTES * sizeof(float), hipMemcpyDeviceToHost, s));
     if (v->reading) {
         FA_HIP_TRY(hipMemcpyAsync(v->h_reward.data(), v->d_reward, v->n * sizeof(float), hipMemcpyDeviceToHost, s));
         FA_HIP_TRY(hipMemcpyAsync(v->h_term.data(), v->d_term, v->n, hipMemcpyDeviceToHost, s));
     }
     FA_HIP_TRY(hipStreamSynchronize(s));
+    if (obs_out) fa::copy_host(obs_out, v->h_obs, v->n * FA_OBS_BYTES * sizeof(float));
     return FA_OK;
 }
 
@@ -145,7 +146,8 @@ int fa_vecenv_create(const uint8_t *rom, size_t rom_len, size_t n, fa_vecenv **o
         return rc;
     }
     if (hipMalloc(&v->d_actions, n) != hipSuccess || hipMalloc(&v->d_fs, n) != hipSuccess ||
-        hipMalloc(&v->d_obs, n * FA_OBS_BYTES * sizeof(float)) != hipSuccess) {
+        hipHostMalloc(&v->h_obs, n * FA_OBS_BYTES * sizeof(float), hipHostMallocDefault) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&v->hv_obs, v->h_obs, 0) != hipSuccess) {
         fa_vecenv_destroy(v);
         return fail(FA_ENOMEM, "device allocation failed");
     }
@@ -156,9 +158,10 @@ int fa_vecenv_create(const uint8_t *rom, size_t rom_len, size_t n, fa_vecenv **o
 void fa_vecenv_destroy(fa_vecenv *v) {
     if (!v) return;
     if (v->env) hipStreamSynchronize(fa::env_stream(v->env));
-    for (void *p : {(void *)v->d_actions, (void *)v->d_fs, (void *)v->d_obs, (void *)v->d_score,
-                    (void *)v->d_reward, (void *)v->d_term})
+    for (void *p : {(void *)v->d_actions, (void *)v->d_fs, (void *)v->d_score, (void *)v->d_reward,
+                    (void *)v->d_term})
         if (p) hipFree(p);
+    if (v->h_obs) hipHostFree(v->h_obs);
     fa_destroy(v->env);
     delete v;
 }
@@ -192,7 +195,7 @@ int fa_vecenv_reset(fa_vecenv *v, float *obs_out) {
     v->steps = 0;
     int rc = fa_reset(v->env, nullptr);
     if (rc) return rc;
-    FA_HIP_TRY(launch_obs_f32(fa::env_obs_device(v->env), v->d_obs, v->n * FA_OBS_BYTES, fa::env_stream(v->env)));
+    FA_HIP_TRY(launch_obs_f32(fa::env_obs_device(v->env), v->hv_obs, v->n * FA_OBS_BYTES, fa::env_stream(v->env)));
     if (v->reading) FA_HIP_TRY(launch_reader(v, 1));
     return finish_host(v, obs_out);
 }
@@ -200,7 +203,7 @@ int fa_vecenv_reset(fa_vecenv *v, float *obs_out) {
 int fa_vecenv_step(fa_vecenv *v, uint32_t action, float *obs_out, float *reward, int *terminated, int *truncated) {
     if (!v) return fail(FA_EINVAL, "NULL handle");
     FA_HIP_TRY(hipMemsetAsync(v->d_actions, (int)(uint8_t)action, v->n, fa::env_stream(v->env)));
-    int rc = launch_step(v, v->d_actions, v->d_obs);
+    int rc = launch_step(v, v->d_actions, v->hv_obs);
     if (rc) return rc;
     rc = finish_host(v, obs_out);
     if (rc) return rc;
@@ -213,7 +216,7 @@ int fa_vecenv_step_each(fa_vecenv *v, const uint8_t *actions, float *obs_out, fl
     if (!v) return fail(FA_EINVAL, "NULL handle");
     if (!actions) return fail(FA_EINVAL, "actions is NULL");
     FA_HIP_TRY(hipMemcpyAsync(v->d_actions, actions, v->n, hipMemcpyHostToDevice, fa::env_stream(v->env)));
-    int rc = launch_step(v, v->d_actions, v->d_obs);
+    int rc = launch_step(v, v->d_actions, v->hv_obs);
     if (rc) return rc;
     rc = finish_host(v, obs_out);
     if (rc) return rc;

@@ -87,6 +87,9 @@ void descriptor_table(uint32_t out[kDescBytes / 4]);
 // holds the obs of device-side steps and of resets that return no host obs)
 hipStream_t env_stream(fa_env *e);
 uint8_t *env_obs_device(fa_env *e);
+// Host memcpy split over up to 8 threads (parts of at least 1 MB): one core
+// copies many megabytes at a fraction of the host's memory bandwidth.
+void copy_host(void *dst, const void *src, size_t bytes);
 
 // Errors: the message of the calling thread's last failure.
 int fail(int code, const std::string &msg);

@@ -153,23 +153,6 @@ int host_obs(fa_env *e, uint8_t **dev) {
     return FA_OK;
 }
 
-void copy_obs(uint8_t *dst, const uint8_t *src, size_t bytes) {
-    constexpr size_t kPart = 1u << 20;
-    const size_t parts = std::min<size_t>(8, (bytes + kPart - 1) / kPart);
-    if (parts <= 1) {
-        std::memcpy(dst, src, bytes);
-        return;
-    }
-    const size_t per = (bytes + parts - 1) / parts;
-    std::vector<std::thread> ts;
-    for (size_t t = 1; t < parts; ++t) {
-        const size_t off = t * per;
-        if (off >= bytes) break;
-        ts.emplace_back([=] { std::memcpy(dst + off, src + off, std::min(per, bytes - off)); });
-    }
-    std::memcpy(dst, src, std::min(per, bytes));
-    for (auto &t : ts) t.join();
-}
 
 // k steps; each holds its action for fs[s * n + i] frames (1 without fs).
 int launch_frames(fa_env *e, const uint8_t *d_actions, const uint8_t *d_fs, uint32_t k, uint8_t *d_obs) {
@@ -222,6 +205,26 @@ int fill_power_on(fa_env *e) {
 
 namespace fa {
 hipStream_t env_stream(fa_env *e) { return e->stream; }
+
+void copy_host(void *dst, const void *src, size_t bytes) {
+    constexpr size_t kPart = 1u << 20;
+    const size_t parts = std::min<size_t>(8, (bytes + kPart - 1) / kPart);
+    auto *d = static_cast<uint8_t *>(dst);
+    auto *s = static_cast<const uint8_t *>(src);
+    if (parts <= 1) {
+        std::memcpy(d, s, bytes);
+        return;
+    }
+    const size_t per = (bytes + parts - 1) / parts;
+    std::vector<std::thread> ts;
+    for (size_t t = 1; t < parts; ++t) {
+        const size_t off = t * per;
+        if (off >= bytes) break;
+        ts.emplace_back([=] { std::memcpy(d + off, s + off, std::min(per, bytes - off)); });
+    }
+    std::memcpy(d, s, std::min(per, bytes));
+    for (auto &t : ts) t.join();
+}
 uint8_t *env_obs_device(fa_env *e) { return e->d_obs; }
 }  // namespace fa
 
@@ -309,7 +312,7 @@ int fa_reset(fa_env *e, uint8_t *obs_out) {
     rc = launch_frames(e, nullptr, nullptr, 1, obs);
     if (rc) return rc;
     FA_HIP_TRY(hipStreamSynchronize(e->stream));
-    if (obs_out) copy_obs(obs_out, e->h_obs, e->n * 128);
+    if (obs_out) fa::copy_host(obs_out, e->h_obs, e->n * 128);
     return FA_OK;
 }
 
@@ -326,7 +329,7 @@ int fa_step_multi(fa_env *e, const uint8_t *actions, size_t n_actions, uint32_t 
     rc = launch_frames(e, e->d_actions, nullptr, k, obs);
     if (rc) return rc;
     FA_HIP_TRY(hipStreamSynchronize(e->stream));
-    if (obs_out) copy_obs(obs_out, e->h_obs, e->n * 128);
+    if (obs_out) fa::copy_host(obs_out, e->h_obs, e->n * 128);
     return FA_OK;
 }
 
@@ -349,7 +352,7 @@ int fa_step_frameskip(fa_env *e, const uint8_t *actions, const uint8_t *fs, size
     rc = launch_frames(e, e->d_actions, e->d_fs, 1, obs);
     if (rc) return rc;
     FA_HIP_TRY(hipStreamSynchronize(e->stream));
-    if (obs_out) copy_obs(obs_out, e->h_obs, e->n * 128);
+    if (obs_out) fa::copy_host(obs_out, e->h_obs, e->n * 128);
     return FA_OK;
 }
 
