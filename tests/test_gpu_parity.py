@@ -282,6 +282,27 @@ def test_hotspot_mid_instruction_switch(fa, orc):
     assert ref.states()[0, 8 + 0x10] == 0          # $90 untouched: JMP target came from bank 1
 
 
+def test_host_obs_of_a_large_batch(fa, W):
+    """Host-side steps at 20,000 envs (2.56 MB of obs, copied out of pinned
+    host memory in several parts): reset, step, step_multi and
+    step_frameskip return exactly each env's RAM (state bytes 8..135)."""
+    n = 20000
+    gpu = fa.BatchAtariGpu(W.rom_for(2), n)
+    acts = W.ActionStream(W.config_seed(2), n)
+
+    def ram():
+        return gpu.download_states_bytes()[:, 8:136]
+
+    assert np.array_equal(gpu.reset().reshape(n, 128), ram())
+    for f in range(3):
+        assert np.array_equal(gpu.step(acts.next()).reshape(n, 128), ram()), f"step {f}"
+    obs = gpu.step_multi(np.stack([acts.next() for _ in range(4)]), 4)
+    assert np.array_equal(obs.reshape(n, 128), ram())
+    obs = gpu.step_frameskip(acts.next(), W.frameskips(7, n))
+    assert np.array_equal(obs.reshape(n, 128), ram())
+    assert not gpu.counters()["status"].any()
+
+
 def test_bad_lengths_raise(fa, W):
     gpu = fa.BatchAtariGpu(W.rom_for(1), 8)
     gpu.reset()

@@ -45,6 +45,23 @@ def test_obs_to_f32_exact_for_every_byte(fa):
     assert got[0] == 0.0 and got[255] == 1.0
 
 
+def test_f32_obs_of_a_large_batch(fa, W):
+    """8,192 envs (4 MB of f32 observations, copied out of pinned host memory
+    in several parts): every observation is the env's RAM / 255 in float32."""
+    n = 8192
+    env = fa.BatchAtariGpuEnv(W.rom_for(2), n).with_frameskip(2)
+
+    def want():
+        ram = env.batch().download_states_bytes()[:, 8:136].reshape(-1)
+        return ram.astype(np.float32) / np.float32(255.0)
+
+    assert _same_bits(env.reset(), want())
+    for a in (3, 9, 17):
+        assert _same_bits(env.step(a).observation, want()), f"action {a}"
+    st = env.step_each(W.ActionStream(11, n).next())
+    assert _same_bits(st.observation, want())
+
+
 @pytest.mark.parametrize("cfg,frameskip", [(2, 4), (3, 2)])
 def test_broadcast_steps_match_oracle(fa, orc, W, cfg, frameskip):
     rom = W.rom_for(cfg)
