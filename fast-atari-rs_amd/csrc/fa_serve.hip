@@ -835,6 +835,18 @@ int fa_loadgen_run(fa_server *s, const int *client_ids, const fa_client_spec *sp
     size_t n_inflight = 0, ns = 0;
     std::vector<uint8_t> act, rst;
     std::vector<int> done;
+    // per env: the first SplitMix64 output of its action and restart streams
+    // (fixed for the run; a step's values are one more mix of these)
+    std::vector<std::vector<uint64_t>> a0s(nclients), r0s(nclients);
+    for (size_t i = 0; i < nclients; ++i) {
+        const fa_client_spec &sp = specs[i];
+        a0s[i].resize(sp.n);
+        r0s[i].resize(sp.n);
+        for (uint32_t e = 0; e < sp.n; ++e) {
+            a0s[i][e] = sm_mix((sp.seed ^ ((uint64_t)e * kGolden)) + kGolden);
+            r0s[i][e] = sm_mix(((sp.seed ^ kRestartSalt) ^ ((uint64_t)e * kGolden)) + kGolden);
+        }
+    }
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->completions.clear();
@@ -869,11 +881,10 @@ int fa_loadgen_run(fa_server *s, const int *client_ids, const fa_client_spec *sp
             act.resize(sp.n);
             rst.resize(sp.n);
             const uint64_t st = steps[i] + 1;
+            const uint64_t *a0 = a0s[i].data(), *r0 = r0s[i].data();
             for (uint32_t e = 0; e < sp.n; ++e) {
-                const uint64_t a0 = sm_mix((sp.seed ^ ((uint64_t)e * kGolden)) + kGolden);
-                const uint64_t r0 = sm_mix(((sp.seed ^ kRestartSalt) ^ ((uint64_t)e * kGolden)) + kGolden);
-                const uint64_t za = sm_mix(a0 + st * kGolden);
-                const uint64_t zr = sm_mix(r0 + st * kGolden);
+                const uint64_t za = sm_mix(a0[e] + st * kGolden);
+                const uint64_t zr = sm_mix(r0[e] + st * kGolden);
                 act[e] = (uint8_t)(((unsigned __int128)za * 18u) >> 64);
                 rst[e] = zr < thresh ? 1 : 0;
             }
