@@ -217,13 +217,19 @@ void copy_host(void *dst, const void *src, size_t bytes) {
     }
     const size_t per = (bytes + parts - 1) / parts;
     std::vector<std::thread> ts;
-    for (size_t t = 1; t < parts; ++t) {
-        const size_t off = t * per;
-        if (off >= bytes) break;
-        ts.emplace_back([=] { std::memcpy(d + off, s + off, std::min(per, bytes - off)); });
+    size_t t = 1;
+    try {
+        for (; t < parts && t * per < bytes; ++t) {
+            const size_t off = t * per;
+            ts.emplace_back([=] { std::memcpy(d + off, s + off, std::min(per, bytes - off)); });
+        }
+    } catch (...) {
+        // no thread to be had: the parts not handed out are copied here
     }
+    for (size_t u = t; u < parts && u * per < bytes; ++u)
+        std::memcpy(d + u * per, s + u * per, std::min(per, bytes - u * per));
     std::memcpy(d, s, std::min(per, bytes));
-    for (auto &t : ts) t.join();
+    for (auto &th : ts) th.join();
 }
 uint8_t *env_obs_device(fa_env *e) { return e->d_obs; }
 }  // namespace fa
