@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <new>
 #include <vector>
 
 #include "fa_internal.hpp"
@@ -138,7 +139,8 @@ int fa_vecenv_create(const uint8_t *rom, size_t rom_len, size_t n, fa_vecenv **o
     if (!out) return fail(FA_EINVAL, "out is NULL");
     *out = nullptr;
     if (n == 0) return fail(FA_EINVAL, "n must be >= 1");
-    auto *v = new fa_vecenv();
+    auto *v = new (std::nothrow) fa_vecenv();
+    if (!v) return fail(FA_ENOMEM, "out of host memory");
     v->n = n;
     int rc = fa_create(rom, rom_len, n, &v->env);
     if (rc) {

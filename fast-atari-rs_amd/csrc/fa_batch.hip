@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -202,7 +203,8 @@ int fa_batch_create(const uint8_t *const *roms, const size_t *rom_lens, const si
     if (!out) return fail(FA_EINVAL, "out is NULL");
     *out = nullptr;
     if (!roms || !rom_lens || !counts || nclass == 0) return fail(FA_EINVAL, "need at least one ROM class");
-    auto *b = new fa_batch();
+    auto *b = new (std::nothrow) fa_batch();
+    if (!b) return fail(FA_ENOMEM, "out of host memory");
     auto cleanup = [&](int code) {
         fa_batch_destroy(b);
         return code;

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <string>
 #include <thread>
 #include <vector>
@@ -153,7 +154,6 @@ int host_obs(fa_env *e, uint8_t **dev) {
     return FA_OK;
 }
 
-
 // k steps; each holds its action for fs[s * n + i] frames (1 without fs).
 int launch_frames(fa_env *e, const uint8_t *d_actions, const uint8_t *d_fs, uint32_t k, uint8_t *d_obs) {
     fa::StepArgs g{};
@@ -244,7 +244,8 @@ int fa_create(const uint8_t *rom, size_t rom_len, size_t n, fa_env **out) {
     fa::RomClass rc;
     int rc_ = fa::make_rom_class(rom, rom_len, rc);
     if (rc_) return rc_;
-    auto *e = new fa_env();
+    auto *e = new (std::nothrow) fa_env();
+    if (!e) return fail(FA_ENOMEM, "out of host memory");
     e->n = n;
     e->n_pad = ((n + kPadEnvs - 1) / kPadEnvs) * kPadEnvs;
     if (e->n_pad == 0) e->n_pad = kPadEnvs;

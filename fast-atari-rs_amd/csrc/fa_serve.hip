@@ -33,6 +33,7 @@
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <random>
 #include <string>
 #include <thread>
@@ -477,7 +478,8 @@ int fa_server_create(const uint8_t *const *roms, const size_t *rom_lens, size_t 
     if (!out) return fail(FA_EINVAL, "out is NULL");
     *out = nullptr;
     if (!roms || !rom_lens || nclass == 0) return fail(FA_EINVAL, "need at least one ROM class");
-    auto *s = new fa_server();
+    auto *s = new (std::nothrow) fa_server();
+    if (!s) return fail(FA_ENOMEM, "out of host memory");
     auto cleanup = [&](int code) {
         fa_server_destroy(s);
         return code;
