@@ -604,10 +604,17 @@ struct Emu {
             st8(wr & ram ? ram_off(a13) : junk, r);
             const bool io = wr & !rom & !ram;
             if (__ballot(io) != 0) {
-                if (io) {
-                    if (d0 & D_RMWMEM) io_write(a13, v);
-                    io_write(a13, r);
+                // The read-modify-write's first write (the old value) leaves a
+                // trace only on VSYNC (edge), VBLANK (paddle dump), GRP0 / GRP1
+                // (the old-copy shift) and HMOVE (motion); on every other TIA
+                // and RIOT register the second write sets the final state alone.
+                const uint32_t tr = a13 & 0x3Fu;
+                const bool lasting = !(a13 & 0x200u) & ((tr <= 0x01u) | (tr == 0x1Bu) | (tr == 0x1Cu) | (tr == 0x2Au));
+                const bool phantom = io & ((d0 & D_RMWMEM) != 0u) & lasting;
+                if (__ballot(phantom) != 0) {
+                    if (phantom) io_write(a13, v);
                 }
+                if (io) io_write(a13, r);
             }
         }
         return cyc;
