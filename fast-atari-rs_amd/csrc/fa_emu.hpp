@@ -466,6 +466,9 @@ struct Emu {
     // and below the hotspot / vector area the fetches have no side effects
     // and come straight from LDS; elsewhere they go through the bus in the
     // reference's order (opcode, then operands or the phantom read of PC).
+    // JSR's high operand byte is not fetched here: the reference reads it
+    // after the two pushes (src/cpu.rs:983-1005), which may have rewritten it
+    // (PC in RAM) or changed the TIA register it comes from (PC in the TIA).
     __device__ __forceinline__ void fetch(uint32_t &b1, uint32_t &b2, uint4 &d, uint32_t &op) {
         if ((pc & 0x1000u) && (pc & 0xFFFu) < 0xFF0u) {
             const uint32_t o = ROM0 + (bank << 12) + (pc & 0xFFFu);
@@ -481,7 +484,7 @@ struct Emu {
             const bool illegal = (d0 & 3u) == CLS_ILLEGAL;
             b1 = len >= 2u ? bus_read((pc + 1u) & 0xFFFFu) : 0u;
             touch_if(len == 1u && !illegal, (pc + 1u) & 0xFFFFu);
-            b2 = len == 3u ? bus_read((pc + 2u) & 0xFFFFu) : 0u;
+            b2 = (len == 3u && op != 0x20u) ? bus_read((pc + 2u) & 0xFFFFu) : 0u;
         }
     }
 
@@ -650,10 +653,12 @@ struct Emu {
             break;
         }
         case SP_JSR: {
+            // the high operand byte is read after the pushes (src/cpu.rs:996-1003);
+            // inside the ROM window this is the byte fetch() would have read
             const uint32_t ret = (pc - 1u) & 0xFFFFu;       // address of the high operand byte
             push(ret >> 8);
             push(ret & 0xFFu);
-            pc = b1 | (b2 << 8);
+            pc = b1 | (bus_read(ret) << 8);
             break;
         }
         case SP_RTS: {

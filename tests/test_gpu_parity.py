@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from gpu_util import assert_same
+from romkit import jsr_rom
 
 pytestmark = pytest.mark.gpu
 
@@ -280,6 +281,44 @@ def test_hotspot_mid_instruction_switch(fa, orc):
         ref.run(a[None, :])
         assert_same(gpu, ref, f"frame {f}")
     assert ref.states()[0, 8 + 0x10] == 0          # $90 untouched: JMP target came from bank 1
+
+
+@pytest.mark.parametrize("sorted_", [False, True])
+def test_jsr_outside_the_rom_window(fa, orc, sorted_):
+    """JSR run from RAM and from TIA space (the per-lane general path): the
+    high operand byte is read after both pushes (src/cpu.rs:983-1005).  JSR
+    at $F0 with SP = $F3 pushes its PCL onto the operand ($F2 -> target
+    $F2C0); with SP = $F2 the PCH lands there ($00 -> $00C0); SP = $F1 and
+    $FF leave it ($D0C0).  In TIA space (PC = $0000, opcode and low operand
+    from CXM0P / CXM1P), SP = $2C makes the first push a CXCLR write, so the
+    high byte read from CXP0FB is 0 after the pushes."""
+    rom = jsr_rom()
+    n = 160
+    gpu, ref = _pair(fa, orc, rom, n, sorted_=sorted_)
+    s = ref.states().copy()
+    case = np.arange(n) % 5
+    sp_of = {0: 0xF3, 1: 0xF2, 2: 0xF1, 3: 0xFF, 4: 0xF3}
+    for i in range(n):
+        s[i, 8 + 0x10] = sp_of[int(case[i])]                 # RAM $90
+        if case[i] == 4:
+            s[i, 3] = 0x2C                                   # SP
+            s[i, 4:6] = 0                                    # PC = $0000
+            s[i, 160:163] = [0x20, 0xC0, 0xD0]               # CXM0P, CXM1P, CXP0FB: JSR $D0C0
+    ref.set_states(s)
+    gpu.upload_states(s)
+    expect = {0: (1, 0, 0), 1: (0, 1, 0), 2: (0, 0, 1), 3: (0, 0, 1), 4: (1, 0, 0)}
+    for f in range(1, 4):
+        a = np.zeros(n, np.uint8)
+        gpu.step(a)
+        ref.run(a[None, :])
+        assert_same(gpu, ref, f"frame {f}")
+        st = ref.states()
+        for c, (m1, m2, m3) in expect.items():
+            row = st[case == c]
+            extra = 1 if c == 4 else 0                       # the TIA-space JSR of the first frame -> $82
+            assert (row[:, 8 + 1] == m1 * f).all(), f"case {c} $81"
+            assert (row[:, 8 + 2] == m2 * f + extra).all(), f"case {c} $82"
+            assert (row[:, 8 + 3] == m3 * f).all(), f"case {c} $83"
 
 
 def test_host_obs_of_a_large_batch(fa, W):
