@@ -4,8 +4,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
@@ -206,30 +209,89 @@ int fill_power_on(fa_env *e) {
 namespace fa {
 hipStream_t env_stream(fa_env *e) { return e->stream; }
 
+// Persistent copy workers: started once for the process (and never stopped:
+// they sleep on a condition variable between copies), so a host step pays no
+// thread start-up.  One copy uses the pool at a time; a concurrent caller
+// copies alone.
+namespace {
+class CopyPool {
+  public:
+    static constexpr size_t kWorkers = 7;
+    static CopyPool *get() {
+        static CopyPool *pool = new CopyPool();     // left running until the process ends
+        return pool;
+    }
+    // Copies `parts` slices of `per` bytes (the last one shorter); false when
+    // the pool is in use or has no workers.
+    bool copy(uint8_t *d, const uint8_t *s, size_t bytes, size_t per, size_t parts) {
+        std::unique_lock<std::mutex> use(use_mu_, std::try_to_lock);
+        if (!use.owns_lock() || nworkers_ == 0) return false;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            d_ = d;
+            s_ = s;
+            bytes_ = bytes;
+            per_ = per;
+            parts_ = parts;
+            next_.store(0);
+            active_ = nworkers_;
+            ++gen_;
+        }
+        cv_work_.notify_all();
+        run_parts();
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_done_.wait(lk, [&] { return active_ == 0; });
+        return true;
+    }
+
+  private:
+    CopyPool() {
+        for (size_t i = 0; i < kWorkers; ++i) {
+            try {
+                std::thread(&CopyPool::worker, this).detach();
+                ++nworkers_;
+            } catch (...) {
+                break;
+            }
+        }
+    }
+    void run_parts() {
+        for (size_t p = next_.fetch_add(1); p < parts_; p = next_.fetch_add(1)) {
+            const size_t off = p * per_;
+            if (off < bytes_) std::memcpy(d_ + off, s_ + off, std::min(per_, bytes_ - off));
+        }
+    }
+    void worker() {
+        uint64_t seen = 0;
+        while (true) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_work_.wait(lk, [&] { return gen_ != seen; });
+                seen = gen_;
+            }
+            run_parts();
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--active_ == 0) cv_done_.notify_all();
+        }
+    }
+    std::mutex use_mu_, mu_;
+    std::condition_variable cv_work_, cv_done_;
+    uint64_t gen_ = 0;
+    size_t nworkers_ = 0, active_ = 0;
+    uint8_t *d_ = nullptr;
+    const uint8_t *s_ = nullptr;
+    size_t bytes_ = 0, per_ = 0, parts_ = 0;
+    std::atomic<size_t> next_{0};
+};
+}  // namespace
+
 void copy_host(void *dst, const void *src, size_t bytes) {
     constexpr size_t kPart = 1u << 20;
-    const size_t parts = std::min<size_t>(8, (bytes + kPart - 1) / kPart);
+    const size_t parts = std::min<size_t>(CopyPool::kWorkers + 1, (bytes + kPart - 1) / kPart);
     auto *d = static_cast<uint8_t *>(dst);
     auto *s = static_cast<const uint8_t *>(src);
-    if (parts <= 1) {
+    if (parts <= 1 || !CopyPool::get()->copy(d, s, bytes, (bytes + parts - 1) / parts, parts))
         std::memcpy(d, s, bytes);
-        return;
-    }
-    const size_t per = (bytes + parts - 1) / parts;
-    std::vector<std::thread> ts;
-    size_t t = 1;
-    try {
-        for (; t < parts && t * per < bytes; ++t) {
-            const size_t off = t * per;
-            ts.emplace_back([=] { std::memcpy(d + off, s + off, std::min(per, bytes - off)); });
-        }
-    } catch (...) {
-        // no thread to be had: the parts not handed out are copied here
-    }
-    for (size_t u = t; u < parts && u * per < bytes; ++u)
-        std::memcpy(d + u * per, s + u * per, std::min(per, bytes - u * per));
-    std::memcpy(d, s, std::min(per, bytes));
-    for (auto &th : ts) th.join();
 }
 uint8_t *env_obs_device(fa_env *e) { return e->d_obs; }
 }  // namespace fa

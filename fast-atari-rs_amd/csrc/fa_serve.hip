@@ -96,6 +96,7 @@ struct LaunchCtx {
     fa::HostTask *dv_tasks = nullptr;  // its device address
     size_t task_cap = 0;
     std::vector<int> clients;
+    std::vector<fa::HostTask> parts;   // each client's slot range and buffers, copied under the lock
     bool busy = false;
     Clock::time_point t_launch{};      // when the launch was issued
 };
@@ -130,7 +131,7 @@ struct fa_server {
     uint32_t coalesce_us = 300;
     uint32_t *d_desc = nullptr;
     std::deque<ServerClass> cls;       // deque: ServerClass is move-only
-    std::deque<Client> clients;        // stable addresses: the dispatcher reads them unlocked
+    std::deque<Client> clients;        // indexed under mu only; element references stay valid as it grows
     std::vector<int> free_ids;
     hipStream_t admin = nullptr;       // client set-up and downloads
     uint32_t *d_admin_tasks = nullptr;
@@ -298,6 +299,13 @@ void dispatch_loop(fa_server *s) {
         sc.queue.swap(rest);
         ctx->busy = true;
         Chunk &ch = *sc.chunks[chunk];
+        // the launch's task list is built from a copy taken under the lock:
+        // fa_server_open may grow `clients` meanwhile
+        ctx->parts.clear();
+        for (int id : ctx->clients) {
+            const Client &cl = s->clients[id];
+            ctx->parts.push_back(fa::HostTask{cl.first, cl.n, cl.dv_in, cl.dv_obs});
+        }
         lk.unlock();
 
         int rc = FA_OK;
@@ -322,11 +330,10 @@ void dispatch_loop(fa_server *s) {
         if (rc == FA_OK) {
             // one workgroup per <= bs envs of each client; inputs and obs stay in the client's pinned buffers
             uint32_t t = 0;
-            for (int id : ctx->clients) {
-                const Client &cl = s->clients[id];
-                for (uint32_t off = 0; off < cl.n; off += bs)
-                    ctx->h_tasks[t++] = fa::HostTask{cl.first + off, std::min(bs, cl.n - off), cl.dv_in + off,
-                                                     cl.dv_obs + (size_t)off * 128u};
+            for (const fa::HostTask &cl : ctx->parts) {
+                for (uint32_t off = 0; off < cl.count; off += bs)
+                    ctx->h_tasks[t++] = fa::HostTask{cl.first + off, std::min(bs, cl.count - off), cl.in + off,
+                                                     cl.obs + (size_t)off * 128u};
             }
             fa::StepArgs g = chunk_args(s, sc, ch);
             g.packed = nullptr;
@@ -402,8 +409,12 @@ void finish_launch(LaunchCtx *ctx) {
 
 int check_client(fa_server *s, int id) {
     if (!s) return fail(FA_EINVAL, "NULL server");
-    if (id < 0 || (size_t)id >= s->clients.size() || !s->clients[id].open)
-        return fail(FA_EINVAL, "no open client " + std::to_string(id));
+    bool ok;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        ok = id >= 0 && (size_t)id < s->clients.size() && s->clients[id].open;
+    }
+    if (!ok) return fail(FA_EINVAL, "no open client " + std::to_string(id));
     return FA_OK;
 }
 
@@ -802,6 +813,8 @@ static inline uint64_t sm_mix(uint64_t z) {
     return z ^ (z >> 31);
 }
 static constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
+// How long the load driver waits for in-flight requests after its deadline.
+static constexpr std::chrono::seconds kDrainLimit{30};
 static constexpr uint64_t kRestartSalt = 0x5E5E7000ull;
 
 int fa_loadgen_run(fa_server *s, const int *client_ids, const fa_client_spec *specs, size_t nclients,
@@ -870,6 +883,26 @@ int fa_loadgen_run(fa_server *s, const int *client_ids, const fa_client_spec *sp
                 }
             }
         } else if (n_inflight == 0) {
+            break;
+        } else if (now > t_end + kDrainLimit) {
+            // requests still in flight long after the deadline: name the
+            // oldest launch that has not completed and return (its kernel is
+            // left to finish on its own)
+            std::lock_guard<std::mutex> lk(s->mu);
+            std::string what = "no launch in flight";
+            double age = -1.0;
+            for (size_t c = 0; c < s->cls.size(); ++c) {
+                if (s->cls[c].inflight.empty()) continue;
+                const double a =
+                    std::chrono::duration<double, std::milli>(now - s->cls[c].inflight.front()->t_launch).count();
+                if (a > age) {
+                    age = a;
+                    what = "class " + std::to_string(c) + ": launch issued " + std::to_string((long long)a) +
+                           " ms ago has not completed";
+                }
+            }
+            result = fail(FA_EHIP, "load driver drain timed out (" + std::to_string(n_inflight) +
+                                       " requests in flight); " + what);
             break;
         }
         // submit for every idle client with a waiting arrival

@@ -294,8 +294,13 @@ class BatchAtariGpu:
     def upload_states(self, states):
         s = np.ascontiguousarray(states)
         if s.dtype != STATE_DTYPE:
-            s = np.ascontiguousarray(s, dtype=np.uint8).reshape(self.n, 222)
-        _check(load_library().fa_upload_states(self._h, s.ctypes.data_as(C.c_void_p), self.n))
+            s = np.ascontiguousarray(s, dtype=np.uint8)
+            if s.size != self.n * 222:
+                raise ValueError(f"states: expected {self.n} x 222 bytes, got {s.size} bytes")
+            s = s.reshape(self.n, 222)
+        elif s.size != self.n:
+            raise ValueError(f"states: expected {self.n} states, got {s.size}")
+        _check(load_library().fa_upload_states(self._h, s.ctypes.data_as(C.c_void_p), s.shape[0]))
 
     def counters(self) -> dict:
         n = self.n
@@ -315,8 +320,10 @@ class BatchAtariGpu:
     def step_frameskip(self, actions, fs) -> np.ndarray:
         a = _u8(actions)
         f = _u8(fs)
+        if a.size != f.size:
+            raise ValueError(f"actions ({a.size}) and frameskips ({f.size}) differ in length")
         obs = np.empty(self.n * 128, np.uint8)
-        _check(load_library().fa_step_frameskip(self._h, _p8(a), _p8(f), self.n, _p8(obs)))
+        _check(load_library().fa_step_frameskip(self._h, _p8(a), _p8(f), a.size, _p8(obs)))
         return obs
 
     def step_device(self, d_actions: int, k: int = 1, d_obs: int | None = None):

@@ -15,7 +15,8 @@ use std::ffi::CStr;
 use std::os::raw::{c_char, c_int};
 
 /// Packed per-env state; byte-identical to `AtariStateGpu` of
-/// src/cuda_env.rs:12-38 and to the C struct in include/fastatari.h.
+/// src/cuda_env.rs:12-38 and to the C struct in include/fastatari.h, with the
+/// reference's field names, so callers that name fields compile unchanged.
 #[repr(C)]
 #[derive(Clone, Copy, Debug)]
 pub struct AtariStateGpu {
@@ -25,14 +26,16 @@ pub struct AtariStateGpu {
     pub port_a_output: u8, pub port_a_ddr: u8, pub port_a_input: u8,
     pub port_b_output: u8, pub port_b_ddr: u8, pub port_b_input: u8,
     pub timer_value: u16, pub timer_prescaler: u16, pub timer_prescaler_select: u16,
-    pub timer_underflow: u8, pub _pad1: [u8; 3],
+    pub timer_underflow: u8, pub _pad1: u8, pub _pad2: u8, pub _pad3: u8,
     pub tia_clock: u16, pub tia_scanline: u16,
     pub tia_wsync: u8, pub tia_frame_complete: u8, pub tia_vsync: u8, pub tia_vblank: u8,
-    pub collisions: [u8; 8],
-    pub inpt4: u8, pub inpt5: u8, pub paddle0: u8, pub paddle1: u8,
-    pub paddle_counter: u16, pub paddle_dumped: u8, pub _pad4: u8,
-    pub pos: [u16; 5],
-    pub hm: [u8; 5], pub _pad5: u8,
+    pub cxm0p: u8, pub cxm1p: u8, pub cxp0fb: u8, pub cxp1fb: u8,
+    pub cxm0fb: u8, pub cxm1fb: u8, pub cxblpf: u8, pub cxppmm: u8,
+    pub inpt4: u8, pub inpt5: u8,
+    pub paddle0: u8, pub paddle1: u8, pub paddle_counter: u16,
+    pub paddle_dumped: u8, pub _pad4: u8,
+    pub pos_p0: u16, pub pos_p1: u16, pub pos_m0: u16, pub pos_m1: u16, pub pos_bl: u16,
+    pub hmp0: u8, pub hmp1: u8, pub hmm0: u8, pub hmm1: u8, pub hmbl: u8, pub _pad5: u8,
     pub grp0: u8, pub grp1: u8, pub grp0_old: u8, pub grp1_old: u8,
     pub enam0: u8, pub enam1: u8, pub enabl: u8, pub enabl_old: u8,
     pub vdelp0: u8, pub vdelp1: u8, pub vdelbl: u8,
@@ -45,6 +48,58 @@ pub struct AtariStateGpu {
 }
 
 const _: () = assert!(std::mem::size_of::<AtariStateGpu>() == 222);
+
+use crate::atari::{BankScheme, HeadlessAtari};
+
+impl AtariStateGpu {
+    /// Packs a CPU emulator into the GPU layout (src/cuda_env.rs:43-94): the
+    /// RIOT RAM goes to `ram`, booleans become 0 / 1, the mapper becomes
+    /// `scheme` 0..3 (Fixed, F8, F6, F4) and every pad byte is 0.  Callers
+    /// that build states on the host upload them with
+    /// `BatchAtariGpu::upload_states`.
+    pub fn from_headless(atari: &HeadlessAtari) -> Self {
+        let (cpu, bus) = (&atari.cpu, &atari.bus);
+        let (t, p) = (&bus.tia, &bus.pia);
+        let scheme = match bus.scheme {
+            BankScheme::Fixed => 0,
+            BankScheme::F8 => 1,
+            BankScheme::F6 => 2,
+            BankScheme::F4 => 3,
+        };
+        AtariStateGpu {
+            a: cpu.a, x: cpu.x, y: cpu.y, sp: cpu.sp, pc: cpu.pc, status: cpu.status.0, _pad0: 0,
+            ram: p.ram,
+            port_a_output: p.port_a_output, port_a_ddr: p.port_a_ddr, port_a_input: p.port_a_input,
+            port_b_output: p.port_b_output, port_b_ddr: p.port_b_ddr, port_b_input: p.port_b_input,
+            timer_value: p.timer_value,
+            timer_prescaler: p.timer_prescaler,
+            timer_prescaler_select: p.timer_prescaler_select,
+            timer_underflow: u8::from(p.timer_underflow), _pad1: 0, _pad2: 0, _pad3: 0,
+            tia_clock: t.clock, tia_scanline: t.scanline,
+            tia_wsync: u8::from(t.wsync), tia_frame_complete: u8::from(t.frame_complete),
+            tia_vsync: t.vsync, tia_vblank: t.vblank,
+            cxm0p: t.cxm0p, cxm1p: t.cxm1p, cxp0fb: t.cxp0fb, cxp1fb: t.cxp1fb,
+            cxm0fb: t.cxm0fb, cxm1fb: t.cxm1fb, cxblpf: t.cxblpf, cxppmm: t.cxppmm,
+            inpt4: u8::from(t.inpt4), inpt5: u8::from(t.inpt5),
+            paddle0: t.paddle0, paddle1: t.paddle1, paddle_counter: t.paddle_counter,
+            paddle_dumped: u8::from(t.paddle_dumped), _pad4: 0,
+            pos_p0: t.pos_p0, pos_p1: t.pos_p1, pos_m0: t.pos_m0, pos_m1: t.pos_m1, pos_bl: t.pos_bl,
+            hmp0: t.hmp0, hmp1: t.hmp1, hmm0: t.hmm0, hmm1: t.hmm1, hmbl: t.hmbl, _pad5: 0,
+            grp0: t.grp0, grp1: t.grp1, grp0_old: t.grp0_old, grp1_old: t.grp1_old,
+            enam0: u8::from(t.enam0), enam1: u8::from(t.enam1),
+            enabl: u8::from(t.enabl), enabl_old: u8::from(t.enabl_old),
+            vdelp0: u8::from(t.vdelp0), vdelp1: u8::from(t.vdelp1), vdelbl: u8::from(t.vdelbl),
+            resmp0: u8::from(t.resmp0), resmp1: u8::from(t.resmp1),
+            nusiz0: t.nusiz0, nusiz1: t.nusiz1,
+            colup0: t.colup0, colup1: t.colup1, colupf: t.colupf, colubk: t.colubk,
+            ctrlpf: t.ctrlpf, refp0: u8::from(t.refp0), refp1: u8::from(t.refp1),
+            pf0: t.pf0, pf1: t.pf1, pf2: t.pf2,
+            bank: bus.bank as u8,
+            scheme,
+            _pad_end: [0; 2],
+        }
+    }
+}
 
 #[repr(C)]
 pub struct FaEnv {
@@ -152,6 +207,7 @@ impl BatchAtariGpu {
 impl BatchAtariGpu {
     /// Replace every env's state (checkpoint restore).
     pub fn upload_states(&mut self, states: &[AtariStateGpu]) -> Result<(), HipError> {
+        assert_eq!(states.len(), self.n);
         check(unsafe { fa_upload_states(self.env, states.as_ptr(), states.len()) })
     }
 
@@ -164,6 +220,8 @@ impl BatchAtariGpu {
 
     /// Env i holds actions[i] for fs[i] frames; obs after each env's last frame.
     pub fn step_frameskip(&mut self, actions: &[u8], fs: &[u8]) -> Result<Vec<u8>, HipError> {
+        assert_eq!(actions.len(), self.n);
+        assert_eq!(fs.len(), self.n);
         let mut obs = vec![0u8; self.n * 128];
         check(unsafe { fa_step_frameskip(self.env, actions.as_ptr(), fs.as_ptr(), actions.len(), obs.as_mut_ptr()) })?;
         Ok(obs)
@@ -186,6 +244,7 @@ unsafe impl Send for BatchMixGpu {}
 
 impl BatchMixGpu {
     pub fn new(roms: &[Vec<u8>], counts: &[usize]) -> Result<Self, HipError> {
+        assert_eq!(counts.len(), roms.len());
         let ptrs: Vec<*const u8> = roms.iter().map(|r| r.as_ptr()).collect();
         let lens: Vec<usize> = roms.iter().map(|r| r.len()).collect();
         let mut b = std::ptr::null_mut();
@@ -203,6 +262,9 @@ impl BatchMixGpu {
     /// k steps: actions / fs laid out [s * N + i]; obs after the last step.
     pub fn step(&mut self, actions: &[u8], fs: Option<&[u8]>, k: u32) -> Result<Vec<u8>, HipError> {
         assert_eq!(actions.len(), k as usize * self.n);
+        if let Some(f) = fs {
+            assert_eq!(f.len(), k as usize * self.n);
+        }
         let mut obs = vec![0u8; self.n * 128];
         let fsp = fs.map_or(std::ptr::null(), |f| f.as_ptr());
         check(unsafe { fa_batch_step(self.b, actions.as_ptr(), fsp, k, 0, obs.as_mut_ptr()) })?;
@@ -211,6 +273,10 @@ impl BatchMixGpu {
 
     /// Autotune block size / grouping / steps per launch per ROM class.
     pub fn tune(&mut self, actions: &[u8], fs: Option<&[u8]>, k: u32) -> Result<(), HipError> {
+        assert_eq!(actions.len(), k as usize * self.n);
+        if let Some(f) = fs {
+            assert_eq!(f.len(), k as usize * self.n);
+        }
         let (mut da, mut df, mut d_obs) = (std::ptr::null_mut(), std::ptr::null_mut(), std::ptr::null_mut());
         check(unsafe { fa_batch_buffers(self.b, k, 0, &mut da, &mut df, &mut d_obs) })?;
         check(unsafe { fa_memcpy_h2d(da, actions.as_ptr(), actions.len()) })?;
