@@ -41,6 +41,7 @@ SOURCES = [
     ("fa_batch.hip", []),
     ("fa_serve.hip", []),
     ("fa_adapter.hip", []),
+    ("fa_noalloc.hip", []),
 ]
 
 

@@ -160,6 +160,7 @@ def load_library():
         "fa_vecenv_step_device": (C.c_int, [vp, vp, C.c_uint32, vp, C.POINTER(C.c_int)]),
         "fa_vecenv_sync": (C.c_int, [vp]),
         "fa_obs_to_f32": (C.c_int, [vp, vp, sz]),
+        "fa_step_noalloc": (C.c_int, [u8p, sz, vp, vp, vp, sz, C.c_uint32, vp, vp]),
         "fa_vecenv_set_reader": (C.c_int, [vp, vp]),
         "fa_vecenv_rewards": (C.c_int, [vp, C.POINTER(C.c_float), u8p]),
         "fa_server_memory": (C.c_int, [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]),
@@ -233,6 +234,22 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+NOALLOC_MAX_ENVS = 65536
+
+
+def step_noalloc(rom, d_states: int, n: int, k: int = 1, d_actions: int | None = None, d_obs: int | None = None,
+                 d_status: int | None = None, stream: int | None = None):
+    """fa_step_noalloc: k frames on n envs whose states are in caller device
+    memory (device pointers as ints), with no allocation by the library;
+    enqueued on `stream` (NULL stream by default), returns without waiting."""
+    r = _u8(rom)
+    _check(load_library().fa_step_noalloc(_p8(r), r.size, C.c_void_p(d_states),
+                                          C.c_void_p(d_status) if d_status else None,
+                                          C.c_void_p(d_actions) if d_actions else None, int(n), int(k),
+                                          C.c_void_p(d_obs) if d_obs else None,
+                                          C.c_void_p(stream) if stream else None))
 
 
 class BatchAtariGpu:

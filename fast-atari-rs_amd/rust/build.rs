@@ -35,6 +35,7 @@ fn build_hip() {
         ("fa_batch.hip", &[]),
         ("fa_serve.hip", &[]),
         ("fa_adapter.hip", &[]),
+        ("fa_noalloc.hip", &[]),
     ];
     let mut objs = Vec::new();
     for (src, extra) in sources.iter() {

@@ -330,6 +330,26 @@ int fa_vecenv_set_reader(fa_vecenv *v, const fa_ram_reader *reader);
 /* Per-env reward and terminated flag of the last reset / step (host, n each). */
 int fa_vecenv_rewards(fa_vecenv *v, float *rewards, uint8_t *terminated);
 
+/* ---- stepping without allocation ----
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): fa_step_device's k-frame step on n envs whose states
+ * (AtariStateGpu[n]) live in the caller's device memory.  d_actions: k * n
+ * bytes laid out [f * n + i] (NULL: no set_action); d_obs: n * 128 bytes, RAM
+ * after the last frame (may be NULL); d_status: n FA_ENV_* words, read and
+ * written back (NULL: every env starts at FA_ENV_OK; an env with a non-zero
+ * status does not run).  ROM as fa_create's.  The descriptors, the ROM image,
+ * the staged state block and the counters live in a fixed device array inside
+ * the library (FA_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends):
+ * n above FA_NOALLOC_MAX_ENVS is FA_EINVAL, and the call makes no device
+ * allocation (no hipMalloc / hipFree or allocating library call).  Enqueued on `stream` (a hipStream_t; NULL = the null
+ * stream), returns without waiting.  Calls that overlap in time on one GPU
+ * share the array and race; the first call on a device sets it up. */
+#define FA_NOALLOC_MAX_ENVS 65536
+#define FA_NOALLOC_SCRATCH_BYTES 16421120
+int fa_step_noalloc(const uint8_t *rom, size_t rom_len, AtariStateGpu *d_states, uint32_t *d_status,
+                    const uint8_t *d_actions, size_t n, uint32_t k, uint8_t *d_obs, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
