@@ -3,10 +3,17 @@
 Workload (one "step" = one frame of every env): the seeded 8 KB F8
 bankswitched WSYNC ROM (bank switch every 16 scanlines), 65,536 envs,
 per-env SplitMix64 action streams, actions resident in HBM before timing.
-`value` is the plain per-frame path (fa_step_device, k = 1); the batched path
-(step_multi, K = 16 frames per launch) is reported beside it.  The CPU
-baseline is the oracle (the C restatement of the reference emulator) on a
-bounded sample of the same workload on the host's cores.
+`value` is the plain per-frame path (fa_step_device, k = 1) over exactly
+--steps frames; the batched path (step_multi, K = 16 frames per launch) is
+timed separately beside it.  `configs` adds config 1 (the Fixed 4 KB ROM at
+1,000 and 65,536 envs) and config 3 (the branchy 16 KB F6 ROM at 65,536 envs,
+plain and with opcode grouping).  `valu_peak` is the device's VALU issue
+peak measured in the same run (fa_probe_valu_peak); the VALU-issue roofline
+divides the kernel's SQ_INSTS_VALU per launch (rocprofv3 --pmc, recorded in
+profiles/ for the current kernel sources only) by the live kernel time.  The
+CPU baseline is the oracle (the C restatement of the reference emulator),
+built -O3 -march=native on the measuring host, at 1, 2, 4, ... threads up to
+the CPU share of one GPU, on the headline ROM and on the branchy ROM.
 
 The same line carries the heterogeneous batch ("mix": four ROM classes in
 one autotuned hipGraph launch of K = 8 steps with per-env frameskip) and the
@@ -44,7 +51,8 @@ HBM_PEAK_GBS = 8000.0
 # cpu cycles, frame cycles, instructions as u64, status as u32).
 BYTES_PER_ENV_FRAME = (222 + 1 + 28) + (222 + 128 + 28)
 N_ENVS = 65536
-# VALU issue peak: 256 CUs x 4 SIMDs x 2.4 GHz / 4 clocks per wave64 VALU op.
+# Nominal VALU issue peak: 256 CUs x 4 SIMDs x 2.4 GHz / 4 clocks per wave64
+# VALU op; bench.py measures the device's own figure (fa_probe_valu_peak).
 VALU_PEAK_G = 256 * 4 * 2.4 / 4
 # HBM bytes and VALU wave-instructions per launch measured with rocprofv3
 # --pmc for the default workload (profiles/*.md give the commands).
@@ -72,16 +80,6 @@ def calibrate_copy(mib):
         y.copy_(x)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes": n * 4, "dispatches": 3}), flush=True)
-
-
-def cpu_baseline(rom, n_sample, frames, seed, threads):
-    from oracle import oracle
-    from fastatari import workloads as W
-    b = oracle.Batch(rom, n_sample, nthreads=threads)
-    b.reset()
-    acts = W.ActionStream(seed, n_sample).frames(frames)
-    secs = b.bench(acts, threads)
-    return n_sample * frames / secs, secs
 
 
 def bench_mix(args, fastatari, W):
@@ -226,19 +224,137 @@ def bench_serve(args, fastatari, W):
     }
 
 
+def kernel_source_hash():
+    """Hash of everything that shapes the frame kernel's code: its sources and
+    the build options.  Profile counters recorded for another hash are not
+    used."""
+    import hashlib
+    h = hashlib.sha256()
+    csrc = os.path.join(ROOT, "fast-atari-rs_amd", "csrc")
+    for name in sorted(os.listdir(csrc)):
+        if name.startswith(("fa_emu", "fa_uniform", "fa_frame", "fa_decode", "fa_kernels", "fa_internal")):
+            with open(os.path.join(csrc, name), "rb") as f:
+                h.update(name.encode() + b"\0" + f.read())
+    with open(os.path.join(ROOT, "fast-atari-rs_amd", "fastatari", "build.py"), "rb") as f:
+        h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def profile_figures(workload, n):
+    """rocprofv3 counter figures of a workload at n envs, if they were
+    recorded for the current kernel sources."""
+    if not os.path.exists(PROFILE_FIGURES):
+        return {}
+    with open(PROFILE_FIGURES) as f:
+        m = json.load(f).get(workload, {})
+    if m.get("envs") != n or m.get("source_hash") != kernel_source_hash():
+        return {}
+    return m
+
+
+def time_plain(gpu, d_act, n, first, frames, d_obs):
+    """`frames` per-frame launches (k = 1) on actions resident in HBM: wall
+    frames/s and the mean kernel time (HIP events on the handle's stream)."""
+    gpu.sync()
+    gpu.kernel_time(reset=True)
+    t0 = time.perf_counter()
+    for f in range(frames):
+        gpu.step_device(d_act.at((first + f) * n), 1, d_obs.ptr)
+    gpu.sync()
+    wall = time.perf_counter() - t0
+    ms, launches = gpu.kernel_time(reset=True)
+    return n * frames / wall, wall, ms / max(launches, 1)
+
+
+def bench_config(fastatari, W, cfg, n, warm, frames, sorted_=False):
+    """Per-frame step of a config's ROM: frames/s, kernel time per frame step,
+    emulated instructions per env-frame."""
+    rom = W.rom_for(cfg)
+    acts = W.ActionStream(W.config_seed(cfg), n).frames(warm + frames)
+    d_act = fastatari.DeviceBuffer(acts.nbytes)
+    d_act.upload(acts)
+    d_obs = fastatari.DeviceBuffer(n * 128)
+    gpu = fastatari.BatchAtariGpu(rom, n)
+    gpu.set_sorted(sorted_)
+    gpu.reset()
+    gpu.set_timing(True)
+    for f in range(warm):
+        gpu.step_device(d_act.at(f * n), 1, d_obs.ptr)
+    ins0 = int(gpu.counters()["instructions"].sum())
+    fps, wall, kms = time_plain(gpu, d_act, n, warm, frames, d_obs)
+    c = gpu.counters()
+    out = {"envs": n, "frames": frames, "sorted": bool(sorted_), "fps": round(fps, 1),
+           "ms_per_frame_step": round(wall / frames * 1e3, 4), "kernel_ms_per_frame_step": round(kms, 4),
+           "instructions_per_frame": round((int(c["instructions"].sum()) - ins0) / (n * frames), 1),
+           "env_errors": int((c["status"] != 0).sum())}
+    gpu.close()
+    d_act.free()
+    d_obs.free()
+    return out
+
+
+def host_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return model, os.cpu_count() or 1, avail
+
+
+def cpu_baseline_curve(W, threads_max, rom_cfgs):
+    """The oracle, built -O3 -march=native on this host, timed on the same
+    workloads at 1, 2, 4, ... threads (each thread owns a contiguous slice of
+    envs; harness shape of examples/benchmark.rs:23-94).  Each point runs a
+    fixed per-thread share of envs and frames (about 1-2 s)."""
+    import tempfile
+    from oracle import oracle
+    build = "-O3 -march=native (built on this host)"
+    try:
+        path = oracle.build_native(os.path.join(tempfile.mkdtemp(prefix="orc_"), "liborc_native.so"))
+        oracle.use_library(path)
+    except Exception as e:                                   # noqa: BLE001 - fall back to the portable build
+        build = f"-O3 portable (native build failed: {type(e).__name__})"
+    out = {}
+    for name, cfg, envs_per_thread, frames in rom_cfgs:
+        rom = W.rom_for(cfg)
+        curve = []
+        t = 1
+        while True:
+            n = envs_per_thread * t
+            b = oracle.Batch(rom, n, nthreads=t)
+            b.reset()
+            acts = W.ActionStream(W.config_seed(cfg), n).frames(frames)
+            secs = b.bench(acts, t)
+            curve.append({"threads": t, "fps": round(n * frames / secs, 1), "seconds": round(secs, 2),
+                          "sample": f"{n} envs x {frames} frames"})
+            if t >= threads_max:
+                break
+            t = min(2 * t, threads_max)
+        out[name] = curve
+    return out, build
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=320)
+    ap.add_argument("--steps", type=int, default=320, help="timed frame steps of the headline (plain) path")
     ap.add_argument("--warmup", type=int, default=16)
     ap.add_argument("--k", type=int, default=16, help="frames per launch of the batched path")
     ap.add_argument("--envs", type=int, default=N_ENVS)
     ap.add_argument("--sorted", action="store_true")
     ap.add_argument("--block", type=int, default=0, help="workgroup size (0: automatic)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="most host threads for the CPU baseline (0: the CPU share of one GPU, at most 16)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="f8",
                     help="f8: the headline 8 KB F8 WSYNC ROM; fixed4k / branchy16k: the other ROM classes")
+    ap.add_argument("--no-configs", action="store_true", help="skip the config 1 / config 3 lines")
     ap.add_argument("--no-mix", action="store_true", help="skip the heterogeneous-batch line")
     ap.add_argument("--mix-launches", type=int, default=3)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving line")
@@ -256,17 +372,19 @@ def main():
         sys.exit(2)
 
     import fastatari
+    from fastatari import env as fenv
     from fastatari import workloads as W
 
     cfg = WORKLOADS[args.workload]
     seed = W.config_seed(cfg)
     rom = W.rom_for(cfg)
     n = args.envs
-    steps, warm, k = args.steps, args.warmup, args.k
-    steps = max(k, (steps // k) * k)
+    steps, warm, k = max(1, args.steps), args.warmup, args.k
+    multi_launches = max(4, steps // k)
 
-    log(f"bench: {WORKLOAD_NAMES[cfg]}, {n} envs, warmup {warm}, timed {steps} frames (plain) + {steps} frames (K={k})")
-    acts = W.ActionStream(seed, n).frames(warm + 2 * steps)            # [frames, n]
+    log(f"bench: {WORKLOAD_NAMES[cfg]}, {n} envs, warmup {warm}, timed {steps} frames (plain) + "
+        f"{multi_launches} launches of K={k}")
+    acts = W.ActionStream(seed, n).frames(warm + steps + multi_launches * k)      # [frames, n]
     d_act = fastatari.DeviceBuffer(acts.nbytes)
     d_act.upload(acts)
     d_obs = fastatari.DeviceBuffer(n * 128)
@@ -278,34 +396,23 @@ def main():
     gpu.reset()
     gpu.set_timing(True)
 
-    # ---- plain per-frame path (the drop-in for atari_frame_kernel)
+    # ---- plain per-frame path (the drop-in for atari_frame_kernel): exactly `steps` frames
     for f in range(warm):
         gpu.step_device(d_act.at(f * n), 1, d_obs.ptr)
-    gpu.sync()
-    gpu.kernel_time(reset=True)
-    t0 = time.perf_counter()
-    for f in range(steps):
-        gpu.step_device(d_act.at((warm + f) * n), 1, d_obs.ptr)
-    gpu.sync()
-    t1 = time.perf_counter()
-    plain_ms_total, plain_launches = gpu.kernel_time(reset=True)
-    plain_wall = t1 - t0
-    plain_fps = n * steps / plain_wall
-    kernel_ms = plain_ms_total / max(plain_launches, 1)
+    plain_fps, plain_wall, kernel_ms = time_plain(gpu, d_act, n, warm, steps, d_obs)
 
     # ---- batched path: K frames per launch, state kept on-chip across frames
     t0 = time.perf_counter()
-    for L in range(steps // k):
+    for L in range(multi_launches):
         gpu.step_device(d_act.at((warm + steps + L * k) * n), k, d_obs.ptr)
     gpu.sync()
-    t1 = time.perf_counter()
-    multi_ms_total, multi_launches = gpu.kernel_time(reset=True)
-    multi_wall = t1 - t0
-    multi_fps = n * steps / multi_wall
+    multi_wall = time.perf_counter() - t0
+    multi_ms_total, multi_n = gpu.kernel_time(reset=True)
+    multi_fps = n * k * multi_launches / multi_wall
 
     counters = gpu.counters()
     instr_total = int(counters["instructions"].sum())
-    frames_total = n * (1 + warm + 2 * steps)
+    frames_total = n * (1 + warm + steps + multi_launches * k)
     status_bad = int((counters["status"] != 0).sum())
 
     # ---- API-level rate (host actions in, host obs out each frame, as the reference measures)
@@ -333,14 +440,16 @@ def main():
     io_ms = io_ms_total / max(io_n, 1)
     io_gbs = (BYTES_PER_ENV_FRAME + 1) * n / (io_ms * 1e-3) / 1e9     # + the frameskip byte
     d_fs0.free()
+    gpu.close()
+    d_act.free()
+    d_obs.free()
+
+    # ---- the VALU issue peak, measured (independent v_add_u32 chains, 8 waves per SIMD)
+    peak = fenv.probe_valu_peak()
+    valu_peak = peak["g_wave_instr_per_s"]
 
     achieved_gbs = BYTES_PER_ENV_FRAME * n / (kernel_ms * 1e-3) / 1e9
-    measured = {}
-    if os.path.exists(PROFILE_FIGURES):
-        with open(PROFILE_FIGURES) as f:
-            measured = json.load(f).get(args.workload, {})
-        if measured.get("envs") != n:
-            measured = {}
+    measured = profile_figures(args.workload, n)
     out = {
         "metric": "aggregate frames/sec at 64K envs (plain vs batched); p50/p99 served step latency",
         "value": round(plain_fps, 1),
@@ -355,10 +464,10 @@ def main():
         "dtype": "u8",
         "data": f"synthetic (seeded {len(rom) // 1024} KB ROM, SplitMix64 per-env actions)",
         "config": {"workload": WORKLOAD_NAMES[cfg],
-                   "envs": n, "frames": steps, "sorted": bool(args.sorted), "block": gpu.block_size(),
-                   "max_lds_per_block": gpu.max_lds()},
-        "batched": {"k": k, "fps": round(multi_fps, 1), "ms_per_launch": round(multi_wall / (steps // k) * 1e3, 4),
-                    "kernel_ms_per_launch": round(multi_ms_total / max(multi_launches, 1), 4)},
+                   "envs": n, "frames": steps, "sorted": bool(args.sorted), "block": args.block or "auto"},
+        "batched": {"k": k, "launches": multi_launches, "frames": multi_launches * k, "fps": round(multi_fps, 1),
+                    "ms_per_launch": round(multi_wall / multi_launches * 1e3, 4),
+                    "kernel_ms_per_launch": round(multi_ms_total / max(multi_n, 1), 4)},
         "api_level_fps": round(api_fps, 1),
         "kernel_ms_per_frame_step": round(kernel_ms, 4),
         "emulated_instr_per_s": round(instr_total / frames_total * plain_fps, 1),
@@ -366,26 +475,52 @@ def main():
         "env_errors": status_bad,
         "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved_gbs / HBM_PEAK_GBS, 6), "traffic": measured.get("traffic_bytes")},
+        "valu_peak": {"measured": round(valu_peak, 1), "nominal": VALU_PEAK_G, "unit": "G wave64-instr/s",
+                      "ms_per_launch": round(peak["ms_per_launch"], 3),
+                      "note": "fa_probe_valu_peak: independent v_add_u32 chains, 8 waves per SIMD"},
         "state_io": {"achieved": round(io_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(io_gbs / HBM_PEAK_GBS, 4), "ms_per_launch": round(io_ms, 4),
                      "bytes_per_launch": (BYTES_PER_ENV_FRAME + 1) * n,
                      "note": "zero-frame launches: state + counters in, state + counters + obs out"},
+        "kernel_source_hash": kernel_source_hash(),
     }
     if "valu_wave_instr" in measured:
         valu = measured["valu_wave_instr"] / (kernel_ms * 1e-3) / 1e9
-        out["valu_issue"] = {"achieved": round(valu, 2), "peak": VALU_PEAK_G, "unit": "G wave64-instr/s",
-                             "frac": round(valu / VALU_PEAK_G, 4),
-                             "note": "SQ_INSTS_VALU per launch (rocprofv3, profiles/) / live kernel time"}
+        out["valu_issue"] = {"bound": "valu", "achieved": round(valu, 2), "peak": round(valu_peak, 1),
+                             "unit": "G wave64-instr/s", "frac": round(valu / valu_peak, 4),
+                             "note": "SQ_INSTS_VALU per launch (rocprofv3 --pmc, profiles/, same kernel sources) "
+                                     "/ live kernel time, against the measured peak"}
     if "wave_instr" in measured:
         # every wave-instruction (VALU, SALU, branch, LDS, SMEM) against one
         # issue per SIMD every 4 clocks: the bound of a SIMD's single wave
         iss = measured["wave_instr"] / (kernel_ms * 1e-3) / 1e9
-        out["issue"] = {"achieved": round(iss, 2), "peak": VALU_PEAK_G, "unit": "G wave-instr/s",
-                        "frac": round(iss / VALU_PEAK_G, 4),
-                        "note": "SQ_INSTS_{VALU,SALU,BRANCH,LDS,SMEM} per launch (rocprofv3, profiles/) / live kernel time"}
-    gpu.close()
-    d_act.free()
-    d_obs.free()
+        out["issue"] = {"achieved": round(iss, 2), "peak": round(valu_peak, 1), "unit": "G wave-instr/s",
+                        "frac": round(iss / valu_peak, 4),
+                        "note": "SQ_INSTS_{VALU,SALU,BRANCH,LDS,SMEM} per launch (rocprofv3, profiles/) / live "
+                                "kernel time"}
+    if not measured:
+        out["valu_issue"] = None
+        out["valu_issue_note"] = "no rocprofv3 counters recorded for these kernel sources (hash differs)"
+
+    if not args.no_configs:
+        log("bench: config 1 (Fixed 4 KB, 1,000 envs and 65,536 envs) and config 3 (branchy F6, plain / sorted)")
+        c1 = bench_config(fastatari, W, 1, 1000, 8, 64)
+        c1_64k = bench_config(fastatari, W, 1, 65536, 8, 32)
+        c3 = bench_config(fastatari, W, 3, 65536, 2, 8)
+        c3s = bench_config(fastatari, W, 3, 65536, 1, 3, sorted_=True)
+        c3_fig = profile_figures("branchy16k", 65536)
+        occ = {}
+        if c3_fig:
+            for mode in ("plain", "sorted"):
+                f = c3_fig.get(mode, {})
+                if "exec_occupancy" in f:
+                    occ[mode] = f["exec_occupancy"]
+        out["configs"] = {
+            "1": {"workload": WORKLOAD_NAMES[1], "1000_envs": c1, "65536_envs": c1_64k},
+            "3": {"workload": WORKLOAD_NAMES[3], "plain": c3, "sorted": c3s,
+                  "sorted_vs_plain": round(c3s["fps"] / c3["fps"], 3),
+                  "exec_mask_occupancy": occ or None},
+        }
     if not args.no_adapter:
         log("bench: env adapter (broadcast action, frameskip 4, f32 obs)")
         out["env_adapter"] = bench_adapter(fastatari, rom, n)
@@ -398,13 +533,22 @@ def main():
         ol = out["served"]["open_loop"]
         out["served_p50_ms"], out["served_p99_ms"] = ol["p50_ms"], ol["p99_ms"]
     if not args.no_cpu:
-        # a bounded sample of the same workload, about 10 s on 16 host threads
-        sample_n, sample_frames = 16384, 192
-        cpu_fps, secs = cpu_baseline(rom, sample_n, sample_frames, seed, args.cpu_threads)
-        out["cpu_baseline"] = {"value": round(cpu_fps, 1), "unit": "frames/s", "cores": args.cpu_threads,
-                               "kind": "port",
-                               "sample": f"{sample_n} envs x {sample_frames} frames of the same ROM/actions "
-                                         f"({secs:.1f} s)"}
+        model, nproc, avail = host_info()
+        # the CPU share of one GPU on a shared host: at most 16 threads
+        tmax = args.cpu_threads or min(16, avail)
+        log(f"bench: CPU baseline (oracle, 1..{tmax} threads of {nproc})")
+        curves, build = cpu_baseline_curve(W, tmax, [("f8", cfg if args.workload == "f8" else 2, 256, 96),
+                                                     ("branchy16k", 3, 64, 16)])
+        head = curves["f8"]
+        out["cpu_baseline"] = {"value": head[-1]["fps"], "unit": "frames/s", "cores": head[-1]["threads"],
+                               "kind": "port", "sample": head[-1]["sample"] + " of the headline ROM/actions",
+                               "single_thread_fps": head[0]["fps"], "build": build,
+                               "host": {"cpu_model": model, "nproc": nproc, "affinity": avail,
+                                        "threads_used": head[-1]["threads"]},
+                               "scaling": curves["f8"],
+                               "branchy16k": {"value": curves["branchy16k"][-1]["fps"],
+                                              "single_thread_fps": curves["branchy16k"][0]["fps"],
+                                              "scaling": curves["branchy16k"]}}
     print(json.dumps(out), flush=True)
 
 

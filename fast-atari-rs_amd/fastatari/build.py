@@ -42,6 +42,7 @@ SOURCES = [
     ("fa_serve.hip", []),
     ("fa_adapter.hip", []),
     ("fa_noalloc.hip", []),
+    ("fa_probe.hip", []),
 ]
 
 

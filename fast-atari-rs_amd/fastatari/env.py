@@ -161,6 +161,7 @@ def load_library():
         "fa_vecenv_sync": (C.c_int, [vp]),
         "fa_obs_to_f32": (C.c_int, [vp, vp, sz]),
         "fa_step_noalloc": (C.c_int, [u8p, sz, vp, vp, vp, sz, C.c_uint32, vp, vp]),
+        "fa_probe_valu_peak": (C.c_int, [C.c_uint32, C.c_uint32, C.POINTER(C.c_double), u64p]),
         "fa_vecenv_set_reader": (C.c_int, [vp, vp]),
         "fa_vecenv_rewards": (C.c_int, [vp, C.POINTER(C.c_float), u8p]),
         "fa_server_memory": (C.c_int, [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]),
@@ -250,6 +251,16 @@ def step_noalloc(rom, d_states: int, n: int, k: int = 1, d_actions: int | None =
                                           C.c_void_p(d_actions) if d_actions else None, int(n), int(k),
                                           C.c_void_p(d_obs) if d_obs else None,
                                           C.c_void_p(stream) if stream else None))
+
+
+def probe_valu_peak(iters: int = 4096, reps: int = 5) -> dict:
+    """Measured VALU issue peak (fa_probe_valu_peak): wave64 v_add_u32 per
+    second over the chip."""
+    ms = C.c_double()
+    adds = C.c_uint64()
+    _check(load_library().fa_probe_valu_peak(int(iters), int(reps), C.byref(ms), C.byref(adds)))
+    return {"ms_per_launch": ms.value, "adds_per_launch": adds.value,
+            "g_wave_instr_per_s": adds.value / (ms.value * 1e-3) / 1e9}
 
 
 class BatchAtariGpu:

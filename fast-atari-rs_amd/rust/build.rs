@@ -36,6 +36,7 @@ fn build_hip() {
         ("fa_serve.hip", &[]),
         ("fa_adapter.hip", &[]),
         ("fa_noalloc.hip", &[]),
+        ("fa_probe.hip", &[]),
     ];
     let mut objs = Vec::new();
     for (src, extra) in sources.iter() {

@@ -330,6 +330,13 @@ int fa_vecenv_set_reader(fa_vecenv *v, const fa_ram_reader *reader);
 /* Per-env reward and terminated flag of the last reset / step (host, n each). */
 int fa_vecenv_rewards(fa_vecenv *v, float *rewards, uint8_t *terminated);
 
+/* ---- measurement ----
+ * No reference counterpart.  The device's VALU issue peak, measured: `reps`
+ * launches (after a warm-up) of a kernel of independent v_add_u32 chains with
+ * 8 waves per SIMD and `iters` loop iterations per wave; returns the mean
+ * launch time and the wave64 adds per launch. */
+int fa_probe_valu_peak(uint32_t iters, uint32_t reps, double *ms_per_launch, uint64_t *adds_per_launch);
+
 /* ---- stepping without allocation ----
  * No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

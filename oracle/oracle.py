@@ -27,12 +27,29 @@ def build() -> str:
     return LIB_PATH
 
 
-def lib():
+def build_native(out_path: str) -> str:
+    """Compile the oracle with -O3 -march=native for this host into out_path
+    (the CPU baseline's build; not used by the tests)."""
+    subprocess.run(["make", "-s", "-C", HERE, "native", f"NATIVE_OUT={out_path}"], check=True)
+    return out_path
+
+
+def use_library(path: str):
+    """Load the oracle from `path` (e.g. the native build) for every later
+    call; must precede the first use of lib()."""
+    global _lib
+    if _lib is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    lib(path)
+
+
+def lib(path: str | None = None):
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        path = path or LIB_PATH
+        if path == LIB_PATH and not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(path)
         vp, u8p, u16p, u32p, u64p = (C.c_void_p, C.POINTER(C.c_uint8), C.POINTER(C.c_uint16),
                                      C.POINTER(C.c_uint32), C.POINTER(C.c_uint64))
         sig = {
