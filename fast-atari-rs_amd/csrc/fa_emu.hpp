@@ -466,9 +466,8 @@ struct Emu {
     // and below the hotspot / vector area the fetches have no side effects
     // and come straight from LDS; elsewhere they go through the bus in the
     // reference's order (opcode, then operands or the phantom read of PC).
-    // JSR's high operand byte is not fetched here: the reference reads it
-    // after the two pushes (src/cpu.rs:983-1005), which may have rewritten it
-    // (PC in RAM) or changed the TIA register it comes from (PC in the TIA).
+    // JSR's high operand byte is read here for its side effects (an INTIM
+    // read, a hotspot) and read again after the two pushes (execute()).
     __device__ __forceinline__ void fetch(uint32_t &b1, uint32_t &b2, uint4 &d, uint32_t &op) {
         if ((pc & 0x1000u) && (pc & 0xFFFu) < 0xFF0u) {
             const uint32_t o = ROM0 + (bank << 12) + (pc & 0xFFFu);
@@ -484,7 +483,7 @@ struct Emu {
             const bool illegal = (d0 & 3u) == CLS_ILLEGAL;
             b1 = len >= 2u ? bus_read((pc + 1u) & 0xFFFFu) : 0u;
             touch_if(len == 1u && !illegal, (pc + 1u) & 0xFFFFu);
-            b2 = (len == 3u && op != 0x20u) ? bus_read((pc + 2u) & 0xFFFFu) : 0u;
+            b2 = len == 3u ? bus_read((pc + 2u) & 0xFFFFu) : 0u;
         }
     }
 
@@ -653,12 +652,23 @@ struct Emu {
             break;
         }
         case SP_JSR: {
-            // the high operand byte is read after the pushes (src/cpu.rs:996-1003);
-            // inside the ROM window this is the byte fetch() would have read
+            // The reference reads the high operand byte after the two pushes
+            // (src/cpu.rs:996-1003).  The pushes (stack page: RAM or TIA
+            // registers) can change only a RAM byte or a TIA register there,
+            // and reading either has no side effect, so fetch()'s read stands
+            // for the side effects of a RIOT / ROM byte and the value is
+            // taken again from RAM or the TIA here.
             const uint32_t ret = (pc - 1u) & 0xFFFFu;       // address of the high operand byte
             push(ret >> 8);
             push(ret & 0xFFu);
-            pc = b1 | (bus_read(ret) << 8);
+            const uint32_t a13 = ret & 0x1FFFu;
+            uint32_t hi = b2;
+            if (__ballot((a13 & 0x1000u) == 0u) != 0) {           // some lane's JSR lies outside the ROM
+                const bool ram = (a13 & 0x1E80u) == 0x0080u, tia = (a13 & 0x1080u) == 0u;
+                const uint32_t m = ld8(ram ? ram_off(a13) : tia_read_off(a13));
+                hi = ram ? m : (tia ? tia_read_val(a13, m) : b2);
+            }
+            pc = b1 | (hi << 8);
             break;
         }
         case SP_RTS: {
