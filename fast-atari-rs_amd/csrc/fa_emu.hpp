@@ -899,11 +899,13 @@ struct Emu {
     // opcode, the wave takes the convergent path: a scalar dispatch on the
     // opcode into the fast path specialised with that opcode's compile-time
     // descriptor.
-    // With GROUP, lanes are first split into groups of equal opcode (64-bit
-    // ballot + readfirstlane) and every group takes the convergent path.
-    // Anything the fast path declines -- and divergent waves without GROUP --
-    // goes through the general path (execute()), which replays the
-    // reference's bus accesses one by one.
+    // With GROUP (set_sorted), a wave whose lanes hold at most two distinct
+    // opcodes runs each opcode group on the convergent path in turn (64-bit
+    // ballot + readlane find the groups); with three or more, the lanes share
+    // one pass of the general path, which costs less than one convergent pass
+    // per opcode.  Anything the fast path declines -- and divergent waves
+    // without GROUP -- goes through the general path (execute()), which
+    // replays the reference's bus accesses one by one.
     template <bool GROUP>
     __device__ __forceinline__ uint32_t exec(uint32_t op, uint32_t b1, uint32_t b2) {
         const uint32_t pc0 = pc;
@@ -911,23 +913,56 @@ struct Emu {
         const uint32_t key = fz ? op : 0x100u;
         uint32_t cyc = 0;
         bool done = false;
-        uint64_t rem = __ballot(1);
-        do {
-            const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)__builtin_ctzll(rem));
-            // The lane test compares against an opaque copy of lk: were it
-            // lk itself, the optimiser would substitute the per-lane key for
-            // lk where the two are equal, and the dispatch below would become
-            // a VALU compare tree under exec masks instead of scalar branches.
-            uint32_t lk_cmp = lk;
-            __asm__ volatile("" : "+s"(lk_cmp));
-            const bool mine = key == lk_cmp;
-            const uint64_t grp = __ballot(mine);
+        const uint64_t all = __ballot(1);
+        // The lane tests compare against an opaque copy of the group's key:
+        // were it the key itself, the optimiser would substitute the per-lane
+        // key for it where the two are equal, and the dispatch below would
+        // become a VALU compare tree under exec masks instead of scalar
+        // branches.
+        const uint32_t k1 = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)__builtin_ctzll(all));
+        uint32_t k1c = k1;
+        __asm__ volatile("" : "+s"(k1c));
+        const uint64_t g1 = __ballot(key == k1c);
+        if constexpr (GROUP) {
+            const uint64_t r2 = all & ~g1;
+            uint32_t k2 = k1;
+            uint64_t g2 = 0;
+            if (r2 != 0) {
+                k2 = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)__builtin_ctzll(r2));
+                uint32_t k2c = k2;
+                __asm__ volatile("" : "+s"(k2c));
+                g2 = __ballot(key == k2c);
+            }
+            if ((g1 | g2) == all) {
+#pragma nounroll
+                for (uint32_t q = 0; q < (r2 != 0 ? 2u : 1u); ++q) {
+                    // the scalar dispatch on the group's opcode encloses the
+                    // per-lane group test (a uniform region around a divergent one)
+                    uint32_t lk = q ? k2 : k1;
+                    __asm__ volatile("" : "+s"(lk));
+                    const bool mine = key == lk;
+                    if (lk < 0x100u) {
+                        switch (lk) {
+#define FA_GCASE(OPC)                                                                             \
+                        case OPC: {                                                               \
+                            constexpr uint64_t D = descriptor_of(OPC);                            \
+                            if (mine) done = fast_uniform((uint32_t)D, (uint32_t)(D >> 32), b1, b2, pc0, cyc); \
+                            break;                                                                \
+                        }
+                            FA_FOR_EACH_OPCODE(FA_GCASE)
+#undef FA_GCASE
+                        default: break;
+                        }
+                    }
+                }
+            }
+        } else {
             // The probability hint lets block placement keep the per-lane
             // loop contiguous below the switch (short branches back to the
             // loop top); divergent waves gain ~2 %, converged ones ~0.5 %.
-            const bool fast = lk < 0x100u && (GROUP ? mine : grp == rem);
-            if (GROUP ? fast : __builtin_expect_with_probability(fast, 1, 0.3)) {
-                switch (lk) {
+            const bool fast = k1 < 0x100u && g1 == all;
+            if (__builtin_expect_with_probability(fast, 1, 0.3)) {
+                switch (k1) {
 #define FA_CASE(OPC)                                                                          \
                 case OPC: {                                                                   \
                     constexpr uint64_t D = descriptor_of(OPC);                                \
@@ -939,8 +974,7 @@ struct Emu {
                 default: break;
                 }
             }
-            rem &= ~grp;
-        } while (GROUP && rem);
+        }
         if (!done) {
             uint32_t gb1 = b1, gb2 = b2, gop = op;
             uint4 gd = ld128(DESC0 + op * 16u);

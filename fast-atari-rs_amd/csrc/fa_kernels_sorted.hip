@@ -1,7 +1,7 @@
 // fa_kernels_sorted.hip -- the opcode-grouping instantiation of the frame
 // kernel (set_sorted).  Lanes are served one distinct opcode at a time
-// (64-bit ballot + readlane); its divergent group loop is compiled with the
-// default control-flow structurisation, unlike the plain path.
+// (64-bit ballot + readlane) while the groups are large; a translation unit
+// of its own so that each kernel gets its own launch attributes.
 #include "fa_frame.hpp"
 
 namespace fa {

@@ -25,18 +25,19 @@ FLAGS = [
     "-fno-slp-vectorize",
     "-Wno-unused-result", "-Wno-unused-value", "-pthread",
 ]
-# The plain frame kernel: control flow whose branches are all wave-uniform is
+# Both frame kernels: control flow whose branches are all wave-uniform is
 # left as written (the opcode dispatch and the uniform mode's handlers stay
 # scalar compare-and-branch code instead of flag-steered flow blocks).  The
-# opcode-grouping kernel keeps the default structurisation: its group loop
-# nests uniform regions inside divergent ones, which this option miscompiles.
+# option mishandles uniform regions nested inside divergent ones, so the
+# opcode-grouping loop puts its scalar dispatch around the per-lane group test
+# rather than inside it.
 UNIFORM_SKIP = ["-mllvm", "-structurizecfg-skip-uniform-regions=true"]
 
 # (source, extra options); fa_kernels*.hip hold the kernels, the others are
 # host code over their launchers
 SOURCES = [
     ("fa_kernels.hip", UNIFORM_SKIP),
-    ("fa_kernels_sorted.hip", []),
+    ("fa_kernels_sorted.hip", UNIFORM_SKIP),
     ("fa_lib.hip", []),
     ("fa_batch.hip", []),
     ("fa_serve.hip", []),

@@ -30,7 +30,7 @@ fn build_hip() {
     let uniform_skip: &[&str] = &["-mllvm", "-structurizecfg-skip-uniform-regions=true"];
     let sources: [(&str, &[&str]); 6] = [
         ("fa_kernels.hip", uniform_skip),
-        ("fa_kernels_sorted.hip", &[]),
+        ("fa_kernels_sorted.hip", uniform_skip),
         ("fa_lib.hip", &[]),
         ("fa_batch.hip", &[]),
         ("fa_serve.hip", &[]),
