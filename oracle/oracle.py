@@ -87,6 +87,27 @@ def lib(path: str | None = None):
             "orc_alu_sweep": (None, [C.c_uint8, u16p]),
             "orc_component_run": (None, [u8p, C.c_uint32, u8p, u32p, C.c_uint32, u32p]),
             "orc_pia_sweep": (None, [u16p, u16p, C.c_uint32, u16p, u16p, u8p]),
+            "orc_r_new": (vp, [u8p, C.c_uint32]),
+            "orc_r_free": (None, [vp]),
+            "orc_r_step": (C.c_uint64, [vp, C.c_uint32, C.c_uint32]),
+            "orc_r_get_state": (None, [vp, u8p]),
+            "orc_r_set_state": (None, [vp, u8p]),
+            "orc_r_get_frame": (None, [vp, u8p]),
+            "orc_r_set_frame": (None, [vp, u8p]),
+            "orc_r_get_counters": (None, [vp, u64p]),
+            "orc_r_bus_read": (C.c_uint8, [vp, C.c_uint16]),
+            "orc_r_bus_write": (None, [vp, C.c_uint16, C.c_uint8]),
+            "orc_r_tia_write": (None, [vp, C.c_uint16, C.c_uint8]),
+            "orc_r_tia_ticks": (None, [vp, C.c_uint32]),
+            "orc_rbatch_new": (vp, [u8p, C.c_uint32, C.c_uint32]),
+            "orc_rbatch_free": (None, [vp]),
+            "orc_rbatch_reset": (None, [vp, C.c_uint32]),
+            "orc_rbatch_run": (None, [vp, u8p, C.c_uint32, C.c_uint32]),
+            "orc_rbatch_states": (None, [vp, u8p]),
+            "orc_rbatch_set_states": (None, [vp, u8p]),
+            "orc_rbatch_frames": (None, [vp, u8p]),
+            "orc_rbatch_counters": (None, [vp, u64p]),
+            "orc_rbatch_bench": (C.c_double, [vp, u8p, C.c_uint32, C.c_uint32]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -231,6 +252,110 @@ class Batch:
     def bench(self, actions: np.ndarray, nthreads: int) -> float:
         a = np.ascontiguousarray(actions, dtype=np.uint8)
         return float(lib().orc_batch_bench(self.h, _p(a, C.c_uint8), a.shape[0], nthreads))
+
+
+RSTATE_BYTES = 256
+FRAME_BYTES = 160 * 192
+
+
+class RConsole:
+    """One rendering Atari (src/atari.rs:226-318 over src/tia.rs)."""
+
+    def __init__(self, rom):
+        self._rom = _u8(rom)
+        self.h = lib().orc_r_new(_p(self._rom, C.c_uint8), len(self._rom))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_r_free(self.h)
+            self.h = None
+
+    def step(self, action: int, frames: int = 1) -> int:
+        return int(lib().orc_r_step(self.h, action, frames))
+
+    def state(self) -> np.ndarray:
+        out = np.zeros(RSTATE_BYTES, np.uint8)
+        lib().orc_r_get_state(self.h, _p(out, C.c_uint8))
+        return out
+
+    def set_state(self, s):
+        s = _u8(s)
+        lib().orc_r_set_state(self.h, _p(s, C.c_uint8))
+
+    def frame(self) -> np.ndarray:
+        out = np.zeros(FRAME_BYTES, np.uint8)
+        lib().orc_r_get_frame(self.h, _p(out, C.c_uint8))
+        return out.reshape(192, 160)
+
+    def set_frame(self, f):
+        f = _u8(np.ascontiguousarray(f).reshape(-1))
+        lib().orc_r_set_frame(self.h, _p(f, C.c_uint8))
+
+    def counters(self):
+        out = np.zeros(4, np.uint64)
+        lib().orc_r_get_counters(self.h, _p(out, C.c_uint64))
+        return [int(x) for x in out]
+
+    def bus_read(self, addr: int) -> int:
+        return int(lib().orc_r_bus_read(self.h, addr))
+
+    def bus_write(self, addr: int, v: int):
+        lib().orc_r_bus_write(self.h, addr, v)
+
+    def tia_write(self, addr: int, v: int):
+        """A TIA register write with no bus cycle around it."""
+        lib().orc_r_tia_write(self.h, addr, v)
+
+    def tia_ticks(self, n: int):
+        """n TIA clocks with no CPU."""
+        lib().orc_r_tia_ticks(self.h, n)
+
+
+class RBatch:
+    """N rendering consoles (frames as 160 x 192 palette indices)."""
+
+    def __init__(self, rom, n: int, nthreads: int = 8):
+        self._rom = _u8(rom)
+        self.n = n
+        self.nthreads = nthreads
+        self.h = lib().orc_rbatch_new(_p(self._rom, C.c_uint8), len(self._rom), n)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_rbatch_free(self.h)
+            self.h = None
+
+    def reset(self):
+        """Atari::new + one run_frame per env."""
+        lib().orc_rbatch_reset(self.h, self.nthreads)
+
+    def run(self, actions: np.ndarray):
+        a = np.ascontiguousarray(actions, dtype=np.uint8)
+        frames = a.shape[0] if a.ndim == 2 else 1
+        lib().orc_rbatch_run(self.h, _p(a, C.c_uint8), frames, self.nthreads)
+
+    def states(self) -> np.ndarray:
+        out = np.zeros((self.n, RSTATE_BYTES), np.uint8)
+        lib().orc_rbatch_states(self.h, _p(out, C.c_uint8))
+        return out
+
+    def set_states(self, s: np.ndarray):
+        s = np.ascontiguousarray(s, dtype=np.uint8)
+        lib().orc_rbatch_set_states(self.h, _p(s, C.c_uint8))
+
+    def frames(self) -> np.ndarray:
+        out = np.zeros((self.n, 192, 160), np.uint8)
+        lib().orc_rbatch_frames(self.h, _p(out, C.c_uint8))
+        return out
+
+    def counters(self) -> np.ndarray:
+        out = np.zeros((self.n, 4), np.uint64)
+        lib().orc_rbatch_counters(self.h, _p(out, C.c_uint64))
+        return out
+
+    def bench(self, actions: np.ndarray, nthreads: int) -> float:
+        a = np.ascontiguousarray(actions, dtype=np.uint8)
+        return float(lib().orc_rbatch_bench(self.h, _p(a, C.c_uint8), a.shape[0], nthreads))
 
 
 class Flat:

@@ -1027,3 +1027,6 @@ void orc_pia_sweep(const uint16_t *in, const uint16_t *n, uint32_t k,
         uf_seen[i] = seen;
     }
 }
+
+/* ---------------------------------------------------------------- rendering console */
+#include "orc_render.inc"

@@ -178,6 +178,36 @@ void orc_component_run(const uint8_t *rom, uint32_t rom_len, uint8_t *state,
 void orc_pia_sweep(const uint16_t *in, const uint16_t *n, uint32_t k,
                    uint16_t *out_n, uint16_t *out_1, uint8_t *uf_seen);
 
+/* ---- rendering console (orc_render.inc: src/tia.rs, src/atari.rs:107-318) ----
+ * State: ORC_RSTATE_BYTES = the 222-byte AtariStateGpu layout (tia_clock in
+ * TIA clocks 0..227), then visible_top @224, vblank_top @226, pending_top
+ * @228 (u16), pending_frames @230, hmove_pending @231, inpt4_latch @232,
+ * inpt5_latch @233, input_latch_enabled @234, zero to 255.  Frame: 160 x 192
+ * palette indices. */
+#define ORC_RSTATE_BYTES 256
+#define ORC_FRAME_BYTES (160 * 192)
+void *orc_r_new(const uint8_t *rom, uint32_t rom_len);        /* Atari::new */
+void orc_r_free(void *h);
+uint64_t orc_r_step(void *h, uint32_t action, uint32_t frames); /* set_action + frames x run_frame */
+void orc_r_get_state(void *h, uint8_t *out);
+void orc_r_set_state(void *h, const uint8_t *in);
+void orc_r_get_frame(void *h, uint8_t *out);
+void orc_r_set_frame(void *h, const uint8_t *in);
+void orc_r_get_counters(void *h, uint64_t out[4]);
+uint8_t orc_r_bus_read(void *h, uint16_t addr);
+void orc_r_bus_write(void *h, uint16_t addr, uint8_t v);
+void orc_r_tia_write(void *h, uint16_t addr, uint8_t v);
+void orc_r_tia_ticks(void *h, uint32_t n);
+void *orc_rbatch_new(const uint8_t *rom, uint32_t rom_len, uint32_t n);
+void orc_rbatch_free(void *b);
+void orc_rbatch_reset(void *b, uint32_t nthreads);            /* Atari::new + one run_frame */
+void orc_rbatch_run(void *b, const uint8_t *actions, uint32_t frames, uint32_t nthreads);
+void orc_rbatch_states(void *b, uint8_t *out);
+void orc_rbatch_set_states(void *b, const uint8_t *in);
+void orc_rbatch_frames(void *b, uint8_t *out);
+void orc_rbatch_counters(void *b, uint64_t *out);
+double orc_rbatch_bench(void *b, const uint8_t *actions, uint32_t frames, uint32_t nthreads);
+
 #ifdef __cplusplus
 }
 #endif
