@@ -9,7 +9,9 @@ built library or without a GPU raises.
 `BatchMixGpu` steps several ROM classes in one hipGraph launch (per-class
 launch shapes from its autotuner); `EnvServer` serves step requests of many
 clients with continuous batching; `BatchAtariGpuEnv` is the env adapter over
-BatchAtariGpu (src/rl4burn_env.rs:268-380).
+BatchAtariGpu (src/rl4burn_env.rs:268-380); `BatchAtariRender` steps the
+rendering console (src/atari.rs:226-378) and returns 160 x 192 palette-index
+frames drawn by the GPU's per-clock TIA.
 """
 import os as _os
 
@@ -24,6 +26,7 @@ from .env import (STATE_DTYPE, BatchAtariGpu, Busy, FastAtariError, DeviceBuffer
 from .batch import BatchMixGpu
 from .serve import EnvServer
 from .vecenv import BatchAtariGpuEnv, Step
+from .render import BatchAtariRender
 
-__all__ = ["BatchAtariGpu", "BatchMixGpu", "EnvServer", "BatchAtariGpuEnv", "Step", "Busy", "FastAtariError", "DeviceBuffer", "STATE_DTYPE",
-           "lib_path", "load_library"]
+__all__ = ["BatchAtariGpu", "BatchMixGpu", "EnvServer", "BatchAtariGpuEnv", "Step", "BatchAtariRender", "Busy",
+           "FastAtariError", "DeviceBuffer", "STATE_DTYPE", "lib_path", "load_library"]

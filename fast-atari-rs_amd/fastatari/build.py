@@ -33,8 +33,9 @@ FLAGS = [
 # rather than inside it.
 UNIFORM_SKIP = ["-mllvm", "-structurizecfg-skip-uniform-regions=true"]
 
-# (source, extra options); fa_kernels*.hip hold the kernels, the others are
-# host code over their launchers
+# (source, extra options); fa_kernels*.hip hold the frame kernels, the others
+# are host code over their launchers, each with its own small kernels
+# (fa_render.hip: the rendering console's kernel and C ABI)
 SOURCES = [
     ("fa_kernels.hip", UNIFORM_SKIP),
     ("fa_kernels_sorted.hip", UNIFORM_SKIP),
@@ -44,6 +45,7 @@ SOURCES = [
     ("fa_adapter.hip", []),
     ("fa_noalloc.hip", []),
     ("fa_probe.hip", []),
+    ("fa_render.hip", []),
 ]
 
 

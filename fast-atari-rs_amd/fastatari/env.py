@@ -168,6 +168,25 @@ def load_library():
         "fa_server_set_coalesce_us": (C.c_int, [vp, C.c_uint32]),
         "fa_server_samples": (C.c_int, [vp, C.POINTER(C.c_double), u32p, u32p, u32p, sz, C.POINTER(sz), u64p,
                                         C.c_int]),
+        # rendering console
+        "fa_render_create": (C.c_int, [u8p, sz, sz, C.POINTER(vp)]),
+        "fa_render_destroy": (None, [vp]),
+        "fa_render_num_envs": (sz, [vp]),
+        "fa_render_reset": (C.c_int, [vp, u8p]),
+        "fa_render_step": (C.c_int, [vp, u8p, sz, C.c_uint32, u8p]),
+        "fa_render_step_device": (C.c_int, [vp, vp, C.c_uint32]),
+        "fa_render_sync": (C.c_int, [vp]),
+        "fa_render_frames_device": (C.c_int, [vp, C.POINTER(vp)]),
+        "fa_render_download_frames": (C.c_int, [vp, u8p, sz]),
+        "fa_render_upload_frames": (C.c_int, [vp, u8p, sz]),
+        "fa_render_download_states": (C.c_int, [vp, u8p, sz]),
+        "fa_render_upload_states": (C.c_int, [vp, u8p, sz]),
+        "fa_render_download_counters": (C.c_int, [vp, u64p, u64p, u64p, u32p]),
+        "fa_render_power_on_state": (C.c_int, [u8p, sz, u8p]),
+        "fa_render_set_block_size": (C.c_int, [vp, C.c_uint32]),
+        "fa_render_get_block_size": (C.c_int, [vp, u32p]),
+        "fa_render_set_timing": (C.c_int, [vp, C.c_int]),
+        "fa_render_kernel_time": (C.c_int, [vp, C.POINTER(C.c_double), u64p, C.c_int]),
         "fa_loadgen_run": (C.c_int, [vp, C.POINTER(C.c_int), vp, sz, C.c_double, C.POINTER(C.c_double),
                                      C.c_uint32, C.c_double, C.c_uint64, C.c_uint64, u64p,
                                      C.POINTER(C.c_double), C.POINTER(C.c_double), u32p, sz, C.POINTER(sz),

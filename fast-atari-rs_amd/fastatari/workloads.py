@@ -608,6 +608,130 @@ def gen_branchy_rom(seed: int, rom_size: int = 16384, segs_per_bank: int = 6, bl
     return bytes(out)
 
 
+# TIA graphics registers (write addresses)
+NUSIZ0, NUSIZ1, COLUP0, COLUP1, COLUPF, COLUBK, CTRLPF, REFP0, REFP1 = 0x04, 0x05, 0x06, 0x07, 0x08, 0x09, 0x0A, 0x0B, 0x0C
+PF0, PF1, PF2, RESP0, RESP1, RESM0, RESM1, RESBL = 0x0D, 0x0E, 0x0F, 0x10, 0x11, 0x12, 0x13, 0x14
+GRP0, GRP1, ENAM0, ENAM1, ENABL, HMP0, HMP1, HMM0, HMM1, HMBL = 0x1B, 0x1C, 0x1D, 0x1E, 0x1F, 0x20, 0x21, 0x22, 0x23, 0x24
+VDELP0, VDELP1, VDELBL, RESMP0, RESMP1, HMOVE, HMCLR, CXCLR = 0x25, 0x26, 0x27, 0x28, 0x29, 0x2A, 0x2B, 0x2C
+FRAME_CTR = 0xD0
+LINE_REGS = [GRP0, GRP1, GRP0, GRP1, COLUP0, COLUP1, COLUPF, COLUBK, PF0, PF1, PF2, PF1, PF2, ENAM0, ENAM1,
+             ENABL, NUSIZ0, NUSIZ1, CTRLPF, REFP0, REFP1, VDELP0, VDELP1, VDELBL, RESMP0, RESMP1, HMP0, HMP1,
+             HMM0, HMM1, HMBL]
+
+
+def gen_render_rom(seed: int, bodies: int = 8, vsync: bool = True) -> bytes:
+    """A 4 KB Fixed graphics kernel for the rendering console: every frame
+    positions the five objects with delay loops whose counts come from RAM
+    (RESP0 / RESP1 / RESM0 / RESM1 / RESBL at env-dependent clocks, then
+    HMxx + HMOVE), runs 192 visible lines of `bodies` random line bodies
+    (stores of RAM-derived or immediate values to the graphics registers,
+    mid-line RESPx, HMOVE after WSYNC, a VBLANK pulse inside a line, RAM
+    updates), reads the collision latches, SWCHA and the latched INPT4 into
+    RAM, and clears the latches after every second frame.  vsync=False leaves out VSYNC, so frames end
+    at run_frame's 100,000-cycle cap, anywhere in the picture."""
+    rng = Rng(seed)
+    a = Asm(0xF000, 4096)
+    _reset_code(a, rng)
+    a.label("frame")
+    if vsync:
+        a.op("LDA", "imm", 2)
+        a.op("STA", "zp", VSYNC)
+        for _ in range(3):
+            a.op("STA", "zp", WSYNC)
+        a.op("LDA", "imm", 0)
+        a.op("STA", "zp", VSYNC)
+    a.op("LDA", "imm", 0x42)                 # blank, input latch on
+    a.op("STA", "zp", VBLANK)
+    # object positions: delay loops of RAM-derived length
+    for j, reg in enumerate((RESP0, RESP1, RESM0, RESM1, RESBL)):
+        a.op("STA", "zp", WSYNC)
+        a.op("LDA", "zp", DATA_LO + 3 * j)
+        a.op("AND", "imm", 0x0F)
+        a.op("TAX")
+        a.op("INX")
+        a.label(f"pos{j}")
+        a.op("DEX")
+        a.op("BNE", "rel", f"pos{j}")
+        a.op("STA", "zp", reg)
+    for j, reg in enumerate((HMP0, HMP1, HMM0, HMM1, HMBL)):
+        a.op("LDA", "zp", DATA_LO + 3 * j + 1)
+        a.op("STA", "zp", reg)
+    a.op("STA", "zp", WSYNC)
+    a.op("STA", "zp", HMOVE)
+    for _ in range(28):
+        a.op("STA", "zp", WSYNC)
+    a.op("LDA", "imm", 0)
+    a.op("STA", "zp", VBLANK)
+    # 192 visible lines: `bodies` line bodies looped
+    a.op("LDA", "imm", 192 // bodies)
+    a.op("STA", "zp", LOOP_CTR)
+    a.op("LDY", "imm", 0)
+    a.label("vis")
+    for b in range(bodies):
+        a.op("STA", "zp", WSYNC)
+        if rng.coin(0.25):
+            a.op("STA", "zp", HMOVE)
+        for _ in range(rng.range(3, 7)):
+            reg = rng.choice(LINE_REGS)
+            src = rng.below(4)
+            if src == 0:
+                a.op("LDA", "imm", rng.below(256))
+            elif src == 1:
+                a.op("LDA", "zp", rng.range(DATA_LO, DATA_HI))
+            elif src == 2:
+                a.op("LDA", "aby", rng.range(DATA_LO, DATA_LO + 0x20))
+            else:
+                a.op("TYA")
+                a.op("EOR", "zp", rng.range(DATA_LO, DATA_HI))
+            a.op("STA", "zp", reg)
+        r = rng.below(8)
+        if r == 0:
+            a.op("STA", "zp", rng.choice([RESP0, RESP1, RESM0, RESM1, RESBL]))
+        elif r == 1:
+            a.op("LDA", "imm", 2)
+            a.op("STA", "zp", VBLANK)
+            a.op("NOP")
+            a.op("LDA", "imm", 0)
+            a.op("STA", "zp", VBLANK)
+        elif r == 2:
+            a.op("STA", "zp", HMCLR)
+        a.op("INC", "zp", rng.range(DATA_LO, DATA_LO + 0x20))
+        a.op("INY")
+    a.op("DEC", "zp", LOOP_CTR)
+    a.op("BEQ", "rel", "vis_done")
+    a.op("JMP", "abs", "vis")
+    a.label("vis_done")
+    a.op("LDA", "imm", 0x42)
+    a.op("STA", "zp", VBLANK)
+    # collisions, joystick and the latched fire button into RAM
+    for j, reg in enumerate(range(0x00, 0x08)):
+        a.op("LDA", "zp", reg)
+        a.op("EOR", "zp", DATA_LO + j)
+        a.op("STA", "zp", DATA_LO + j)
+    a.op("LDA", "abs", SWCHA)
+    a.op("EOR", "zp", DATA_LO + 8)
+    a.op("STA", "zp", DATA_LO + 8)
+    a.op("LDA", "zp", INPT4)
+    a.op("ADC", "zp", DATA_LO + 9)
+    a.op("STA", "zp", DATA_LO + 9)
+    a.op("INC", "zp", FRAME_CTR)             # latches cleared after even frames only
+    a.op("LDA", "zp", FRAME_CTR)
+    a.op("LSR", "acc")
+    a.op("BCS", "rel", "keep_cx")
+    a.op("STA", "zp", CXCLR)
+    a.label("keep_cx")
+    a.op("LDX", "imm", 24)
+    a.label("ovs")
+    a.op("STA", "zp", WSYNC)
+    a.op("DEX")
+    a.op("BNE", "rel", "ovs")
+    a.op("JMP", "abs", "frame")
+    rom = bytearray(a.link())
+    for off, val in ((0xFFC, 0x00), (0xFFD, 0xF0), (0xFFE, 0x00), (0xFFF, 0xF0)):
+        rom[off] = val
+    return bytes(rom)
+
+
 # --------------------------------------------------------------------------
 # the five benchmark configurations
 # --------------------------------------------------------------------------

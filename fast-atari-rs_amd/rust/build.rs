@@ -24,11 +24,11 @@ fn build_hip() {
         "-fno-slp-vectorize",
         "-pthread",
     ];
-    // (source, extra options): the plain frame kernel keeps branches whose
-    // conditions are wave-uniform as scalar branches; the opcode-grouping
-    // kernel keeps the default structurisation; the rest is host code.
+    // (source, extra options): both frame kernels keep branches whose
+    // conditions are wave-uniform as scalar branches; the rest is host code
+    // with its own small kernels (fa_render.hip: the rendering console).
     let uniform_skip: &[&str] = &["-mllvm", "-structurizecfg-skip-uniform-regions=true"];
-    let sources: [(&str, &[&str]); 6] = [
+    let sources: [(&str, &[&str]); 9] = [
         ("fa_kernels.hip", uniform_skip),
         ("fa_kernels_sorted.hip", uniform_skip),
         ("fa_lib.hip", &[]),
@@ -37,6 +37,7 @@ fn build_hip() {
         ("fa_adapter.hip", &[]),
         ("fa_noalloc.hip", &[]),
         ("fa_probe.hip", &[]),
+        ("fa_render.hip", &[]),
     ];
     let mut objs = Vec::new();
     for (src, extra) in sources.iter() {

@@ -357,6 +357,61 @@ int fa_probe_valu_peak(uint32_t iters, uint32_t reps, double *ms_per_launch, uin
 int fa_step_noalloc(const uint8_t *rom, size_t rom_len, AtariStateGpu *d_states, uint32_t *d_status,
                     const uint8_t *d_actions, size_t n, uint32_t k, uint8_t *d_obs, void *stream);
 
+/* ---- rendering console: pixel frames ----
+ * A batch of the reference's rendering `Atari` (src/atari.rs:226-378): the
+ * per-clock TIA (src/tia.rs:98-675) drawing 160 x 192 frames of NTSC palette
+ * indices, on the rendering bus whose every CPU access clocks the TIA three
+ * times and the RIOT once (src/atari.rs:107-224).  A frame is one run_frame
+ * (src/atari.rs:277-305); each env's frame is Atari::framebuffer
+ * (src/atari.rs:320-323) after its last frame.  The render state is
+ * FA_RENDER_STATE_BYTES per env: the AtariStateGpu layout (tia_clock counts
+ * TIA clocks 0..227 here), then visible_top (u16 @224), vblank_top (u16 @226),
+ * pending_top (u16 @228), pending_frames @230, hmove_pending @231,
+ * inpt4_latch @232, inpt5_latch @233, input_latch_enabled @234, zeros to 256. */
+typedef struct fa_render fa_render;
+#define FA_FRAME_WIDTH 160
+#define FA_FRAME_HEIGHT 192
+#define FA_FRAME_BYTES (FA_FRAME_WIDTH * FA_FRAME_HEIGHT)
+#define FA_RENDER_STATE_BYTES 256
+#define FA_RENDER_MAX_ENVS (1u << 20)
+
+/* n consoles of one ROM (sizes as fa_create's), each at Atari::new
+ * (src/atari.rs:238-246) with an all-zero frame. */
+int fa_render_create(const uint8_t *rom, size_t rom_len, size_t n, fa_render **out);
+void fa_render_destroy(fa_render *r);
+size_t fa_render_num_envs(const fa_render *r);
+/* Env::reset (src/atari.rs:330-340) on every env: Atari::new, then one
+ * run_frame with no action set.  frames_out: n * FA_FRAME_BYTES (host; NULL:
+ * not copied). */
+int fa_render_reset(fa_render *r, uint8_t *frames_out);
+/* Env::step (src/atari.rs:342-357) k times: frame f sets actions[f * n + i]
+ * (indices above 17 are no-ops, as Action::from_index's default) then runs a
+ * frame.  actions: k * n bytes (host); frames_out as fa_render_reset's. */
+int fa_render_step(fa_render *r, const uint8_t *actions, size_t n_actions, uint32_t k, uint8_t *frames_out);
+/* Device form: actions in device memory (k * n bytes, NULL: none set); queued
+ * on the handle's stream, complete after fa_render_sync. */
+int fa_render_step_device(fa_render *r, const uint8_t *d_actions, uint32_t k);
+int fa_render_sync(fa_render *r);
+/* The handle's device frame array (n * FA_FRAME_BYTES, env i at i * FA_FRAME_BYTES). */
+int fa_render_frames_device(fa_render *r, uint8_t **d_frames);
+int fa_render_download_frames(fa_render *r, uint8_t *out, size_t n);
+int fa_render_upload_frames(fa_render *r, const uint8_t *in, size_t n);
+/* Render states (n * FA_RENDER_STATE_BYTES).  An upload is checked for
+ * reachable values (FA_ESTATE) and clears every env's status. */
+int fa_render_download_states(fa_render *r, uint8_t *out, size_t n);
+int fa_render_upload_states(fa_render *r, const uint8_t *in, size_t n);
+/* Cpu.cycles, the last run_frame's cycles, instructions, FA_ENV_* (any may be NULL). */
+int fa_render_download_counters(fa_render *r, uint64_t *cpu_cycles, uint64_t *frame_cycles, uint64_t *instructions,
+                                uint32_t *status);
+/* Atari::new's state for a ROM (FA_RENDER_STATE_BYTES into out). */
+int fa_render_power_on_state(const uint8_t *rom, size_t rom_len, uint8_t *out);
+/* Workgroup size (0: default 64; else 64, 128, 192 or 256 within the LDS budget). */
+int fa_render_set_block_size(fa_render *r, uint32_t block);
+int fa_render_get_block_size(const fa_render *r, uint32_t *block);
+/* Per-launch HIP event timing on the handle's stream, as fa_set_timing / fa_kernel_time. */
+int fa_render_set_timing(fa_render *r, int enabled);
+int fa_render_kernel_time(fa_render *r, double *total_ms, uint64_t *launches, int reset);
+
 #ifdef __cplusplus
 }
 #endif

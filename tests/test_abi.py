@@ -62,6 +62,8 @@ def test_rust_build_script_compiles_what_the_python_build_compiles():
     pairs = re.findall(r'\("(fa_[a-z_]+\.hip)",\s*(uniform_skip|&\[\])\)', text)
     rust = {src: (build.UNIFORM_SKIP if opt == "uniform_skip" else []) for src, opt in pairs}
     assert rust == {src: list(extra) for src, extra in build.SOURCES}
+    n_decl = re.search(r"let sources: \[\(&str, &\[&str\]\); (\d+)\]", text)
+    assert n_decl and int(n_decl.group(1)) == len(pairs), "array length must match its entries"
     m = re.search(r"let uniform_skip: &\[&str\] = &\[(.*?)\];", text)
     assert m and re.findall(r'"([^"]+)"', m.group(1)) == build.UNIFORM_SKIP
     flags = re.findall(r'"([^"]+)"', re.search(r"let flags = \[(.*?)\];", text, re.S).group(1))
