@@ -20,7 +20,10 @@ one autotuned hipGraph launch of K = 8 steps with per-env frameskip) and the
 served mode ("served": 256 clients with continuous batching; p50 / p99 step
 latency beside aggregate frames/s; its bit-exactness, 1 % of envs against the
 oracle, is checked in tests/test_gpu_serve.py) and the env adapter
-("env_adapter": BatchAtariGpuEnv steps with f32 observations).  "state_io"
+("env_adapter": BatchAtariGpuEnv steps with f32 observations), and the
+rendering console ("render": 160 x 192 palette-index frames drawn by the
+per-clock TIA, with an HBM roofline on the frame bytes and the rendering
+oracle's CPU rate beside it).  "state_io"
 is the HBM rate of the frame-boundary state traffic alone (launches that run
 zero frames per env: state and counters in, state, counters and obs out).
 
@@ -293,6 +296,65 @@ def bench_config(fastatari, W, cfg, n, warm, frames, sorted_=False):
     return out
 
 
+def bench_render(fastatari, W, name, rom, n, warm, k, launches):
+    """The rendering console (fa_render_kernel): k frames per launch of n
+    envs drawing 160 x 192 palette-index frames, actions resident in HBM.
+    HBM roofline on the algorithmic bytes: each env-frame writes its frame
+    (30,720 B) and reads its action; each launch reads and writes the
+    256-byte render state and the counters."""
+    from fastatari.render import FRAME_BYTES, RENDER_STATE_BYTES, BatchAtariRender
+    acts = W.ActionStream(W.config_seed(1), n).frames(warm + k * launches)
+    d_act = fastatari.DeviceBuffer(acts.nbytes)
+    d_act.upload(acts)
+    g = BatchAtariRender(rom, n)
+    g.reset()
+    g.set_timing(True)
+    if warm:
+        g.step_device(d_act.ptr, warm)
+    g.sync()
+    g.kernel_time(reset=True)
+    ins0 = int(g.counters()["instructions"].sum())
+    t0 = time.perf_counter()
+    for L in range(launches):
+        g.step_device(d_act.at((warm + L * k) * n), k)
+    g.sync()
+    wall = time.perf_counter() - t0
+    ms, nl = g.kernel_time(reset=True)
+    kms = ms / max(nl, 1)
+    c = g.counters()
+    frames = n * k * launches
+    alg = n * (k * (FRAME_BYTES + 1) + 2 * (RENDER_STATE_BYTES + 28))
+    achieved = alg / (kms * 1e-3) / 1e9
+    out = {"workload": name, "envs": n, "k": k, "launches": launches, "fps": round(frames / wall, 1),
+           "kernel_ms_per_launch": round(kms, 3), "kernel_ms_per_frame_step": round(kms / k, 3),
+           "instructions_per_frame": round((int(c["instructions"].sum()) - ins0) / frames, 1),
+           "env_errors": int((c["status"] != 0).sum()), "block": g.block_size(),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 6), "bytes_per_launch": alg,
+                        "note": "algorithmic bytes: frame + action per env-frame, render state + counters in "
+                                "and out per launch"}}
+    g.close()
+    d_act.free()
+    return out
+
+
+def render_cpu_baseline(W, threads):
+    """The rendering oracle (native build when loaded) on the config-1 ROM:
+    frames/s at each thread count, each thread 16 envs x 8 frames."""
+    from oracle import oracle
+    rom = W.rom_for(1)
+    pts = []
+    for t in threads:
+        n = 16 * t
+        b = oracle.RBatch(rom, n, nthreads=t)
+        b.reset()
+        acts = W.ActionStream(W.config_seed(1), n).frames(8)
+        secs = b.bench(acts, t)
+        pts.append({"threads": t, "fps": round(n * 8 / secs, 1), "seconds": round(secs, 2),
+                    "sample": f"{n} envs x 8 frames"})
+    return pts
+
+
 def host_info():
     model = "unknown"
     try:
@@ -359,6 +421,7 @@ def main():
     ap.add_argument("--mix-launches", type=int, default=3)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving line")
     ap.add_argument("--no-adapter", action="store_true", help="skip the env-adapter line")
+    ap.add_argument("--no-render", action="store_true", help="skip the rendering-console line")
     ap.add_argument("--serve-seconds", type=float, default=6.0, help="open-loop serving time")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy kernel of MIB MiB (HBM counter calibration)")
@@ -521,6 +584,17 @@ def main():
                   "sorted_vs_plain": round(c3s["fps"] / c3["fps"], 3),
                   "exec_mask_occupancy": occ or None},
         }
+    if not args.no_render:
+        log("bench: rendering console (pixel TIA frames, 65,536 and 1,000 envs)")
+        out["render"] = {
+            "config1_65536": bench_render(fastatari, W, "config-1 ROM (seeded Fixed 4 KB WSYNC kernel)",
+                                          W.rom_for(1), 65536, 1, 4, 2),
+            "graphics_65536": bench_render(fastatari, W, "graphics kernel (seeded 4 KB Fixed, sprites / "
+                                           "playfield / HMOVE / mid-line RESPx)", W.gen_render_rom(100),
+                                           65536, 1, 4, 2),
+            "config1_1000": bench_render(fastatari, W, "config-1 ROM (seeded Fixed 4 KB WSYNC kernel)",
+                                         W.rom_for(1), 1000, 1, 16, 2),
+        }
     if not args.no_adapter:
         log("bench: env adapter (broadcast action, frameskip 4, f32 obs)")
         out["env_adapter"] = bench_adapter(fastatari, rom, n)
@@ -549,6 +623,10 @@ def main():
                                "branchy16k": {"value": curves["branchy16k"][-1]["fps"],
                                               "single_thread_fps": curves["branchy16k"][0]["fps"],
                                               "scaling": curves["branchy16k"]}}
+        if not args.no_render:
+            rc = render_cpu_baseline(W, sorted({1, tmax}))
+            out["cpu_baseline"]["render_config1"] = {"value": rc[-1]["fps"], "single_thread_fps": rc[0]["fps"],
+                                                     "kind": "port", "scaling": rc}
     print(json.dumps(out), flush=True)
 
 
