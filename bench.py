@@ -227,30 +227,39 @@ def bench_serve(args, fastatari, W):
     }
 
 
-def kernel_source_hash():
-    """Hash of everything that shapes the frame kernel's code: its sources and
-    the build options.  Profile counters recorded for another hash are not
-    used."""
+def kernel_source_hash(prefixes=("fa_emu", "fa_uniform", "fa_frame", "fa_decode", "fa_kernels", "fa_internal"),
+                       units=("fa_kernels.hip", "fa_kernels_sorted.hip")):
+    """Hash of everything that shapes a kernel's code: the sources whose names
+    start with `prefixes` and the compile options of its translation `units`.
+    Profile counters recorded for another hash are not used."""
     import hashlib
+    from fastatari import build as fb
     h = hashlib.sha256()
     csrc = os.path.join(ROOT, "fast-atari-rs_amd", "csrc")
     for name in sorted(os.listdir(csrc)):
-        if name.startswith(("fa_emu", "fa_uniform", "fa_frame", "fa_decode", "fa_kernels", "fa_internal")):
+        if name.startswith(prefixes):
             with open(os.path.join(csrc, name), "rb") as f:
                 h.update(name.encode() + b"\0" + f.read())
-    with open(os.path.join(ROOT, "fast-atari-rs_amd", "fastatari", "build.py"), "rb") as f:
-        h.update(f.read())
+    h.update(repr(fb.FLAGS).encode())
+    for src, extra in fb.SOURCES:
+        if src in units:
+            h.update(repr((src, extra)).encode())
     return h.hexdigest()[:16]
 
 
-def profile_figures(workload, n):
+def render_source_hash():
+    """kernel_source_hash of the rendering console's kernel."""
+    return kernel_source_hash(("fa_render", "fa_decode"), ("fa_render.hip",))
+
+
+def profile_figures(workload, n, source_hash=None):
     """rocprofv3 counter figures of a workload at n envs, if they were
     recorded for the current kernel sources."""
     if not os.path.exists(PROFILE_FIGURES):
         return {}
     with open(PROFILE_FIGURES) as f:
         m = json.load(f).get(workload, {})
-    if m.get("envs") != n or m.get("source_hash") != kernel_source_hash():
+    if m.get("envs") != n or m.get("source_hash") != (source_hash or kernel_source_hash()):
         return {}
     return m
 
@@ -296,7 +305,7 @@ def bench_config(fastatari, W, cfg, n, warm, frames, sorted_=False):
     return out
 
 
-def bench_render(fastatari, W, name, rom, n, warm, k, launches):
+def bench_render(fastatari, W, key, name, rom, n, warm, k, launches, valu_peak=None):
     """The rendering console (fa_render_kernel): k frames per launch of n
     envs drawing 160 x 192 palette-index frames, actions resident in HBM.
     HBM roofline on the algorithmic bytes: each env-frame writes its frame
@@ -325,14 +334,28 @@ def bench_render(fastatari, W, name, rom, n, warm, k, launches):
     frames = n * k * launches
     alg = n * (k * (FRAME_BYTES + 1) + 2 * (RENDER_STATE_BYTES + 28))
     achieved = alg / (kms * 1e-3) / 1e9
+    measured = profile_figures(f"render_{key}", n, render_source_hash()) if key else {}
+    if measured.get("k") != k:
+        measured = {}
     out = {"workload": name, "envs": n, "k": k, "launches": launches, "fps": round(frames / wall, 1),
            "kernel_ms_per_launch": round(kms, 3), "kernel_ms_per_frame_step": round(kms / k, 3),
            "instructions_per_frame": round((int(c["instructions"].sum()) - ins0) / frames, 1),
            "env_errors": int((c["status"] != 0).sum()), "block": g.block_size(),
            "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 6), "bytes_per_launch": alg,
+                        "traffic": measured.get("traffic_bytes"),
                         "note": "algorithmic bytes: frame + action per env-frame, render state + counters in "
-                                "and out per launch"}}
+                                "and out per launch; traffic: FETCH_SIZE + WRITE_SIZE per launch (rocprofv3, "
+                                "profiles/)"},
+           "source_hash": render_source_hash()}
+    if "valu_wave_instr" in measured and valu_peak:
+        valu = measured["valu_wave_instr"] / (kms * 1e-3) / 1e9
+        iss = measured["wave_instr"] / (kms * 1e-3) / 1e9
+        out["valu_issue"] = {"bound": "valu", "achieved": round(valu, 2), "peak": round(valu_peak, 1),
+                             "unit": "G wave64-instr/s", "frac": round(valu / valu_peak, 4),
+                             "issue_frac": round(iss / valu_peak, 4),
+                             "note": "SQ_INSTS_VALU (issue_frac: all SQ_INSTS_*) per launch, rocprofv3 --pmc for "
+                                     "the same kernel sources, / live kernel time, against the measured peak"}
     g.close()
     d_act.free()
     return out
@@ -340,18 +363,19 @@ def bench_render(fastatari, W, name, rom, n, warm, k, launches):
 
 def render_cpu_baseline(W, threads):
     """The rendering oracle (native build when loaded) on the config-1 ROM:
-    frames/s at each thread count, each thread 16 envs x 8 frames."""
+    frames/s at each thread count, each thread 96 envs x 16 frames (about
+    0.7 s a point)."""
     from oracle import oracle
     rom = W.rom_for(1)
     pts = []
     for t in threads:
-        n = 16 * t
+        n = 96 * t
         b = oracle.RBatch(rom, n, nthreads=t)
         b.reset()
-        acts = W.ActionStream(W.config_seed(1), n).frames(8)
+        acts = W.ActionStream(W.config_seed(1), n).frames(16)
         secs = b.bench(acts, t)
-        pts.append({"threads": t, "fps": round(n * 8 / secs, 1), "seconds": round(secs, 2),
-                    "sample": f"{n} envs x 8 frames"})
+        pts.append({"threads": t, "fps": round(n * 16 / secs, 1), "seconds": round(secs, 2),
+                    "sample": f"{n} envs x 16 frames"})
     return pts
 
 
@@ -586,14 +610,15 @@ def main():
         }
     if not args.no_render:
         log("bench: rendering console (pixel TIA frames, 65,536 and 1,000 envs)")
+        c1name = "config-1 ROM (seeded Fixed 4 KB WSYNC kernel)"
+        gname = "graphics kernel (seeded 4 KB Fixed, sprites / playfield / HMOVE / mid-line RESPx)"
         out["render"] = {
-            "config1_65536": bench_render(fastatari, W, "config-1 ROM (seeded Fixed 4 KB WSYNC kernel)",
-                                          W.rom_for(1), 65536, 1, 4, 2),
-            "graphics_65536": bench_render(fastatari, W, "graphics kernel (seeded 4 KB Fixed, sprites / "
-                                           "playfield / HMOVE / mid-line RESPx)", W.gen_render_rom(100),
-                                           65536, 1, 4, 2),
-            "config1_1000": bench_render(fastatari, W, "config-1 ROM (seeded Fixed 4 KB WSYNC kernel)",
-                                         W.rom_for(1), 1000, 1, 16, 2),
+            "config1_65536": bench_render(fastatari, W, "config1_65536", c1name, W.rom_for(1), 65536, 1, 4, 2,
+                                          valu_peak),
+            "graphics_65536": bench_render(fastatari, W, "graphics_65536", gname, W.gen_render_rom(100), 65536,
+                                           1, 4, 2, valu_peak),
+            "config1_1000": bench_render(fastatari, W, "config1_1000", c1name, W.rom_for(1), 1000, 1, 16, 2,
+                                         valu_peak),
         }
     if not args.no_adapter:
         log("bench: env adapter (broadcast action, frameskip 4, f32 obs)")
