@@ -41,6 +41,7 @@
 #include <vector>
 
 #include "fa_internal.hpp"
+#include "fa_serve_wait.hpp"
 
 namespace {
 
@@ -52,6 +53,7 @@ constexpr uint32_t kUnit = 64;                 // slot allocation granule
 constexpr uint32_t kUnits = kChunkSlots / kUnit;
 constexpr uint32_t kCtxPerClass = 1;
 constexpr uint32_t kTaskEnvs = 256;            // envs per workgroup of the admin launches
+constexpr double kLaunchLimit = 60.0;          // seconds a completer waits for one launch
 // A class's workgroup size: the largest of 256 / 128 / 64 envs whose LDS
 // footprint stays within kShareLds, so that workgroups of two classes fit on
 // one CU together (160 KB) -- a slow class's long-running workgroups then do
@@ -359,7 +361,8 @@ void dispatch_loop(fa_server *s) {
     }
 }
 
-// Completer of one class: waits for the class's launches in stream order.
+// Completer of one class: waits for the class's launches in stream order,
+// each for at most kLaunchLimit seconds.
 void complete_loop(fa_server *s, size_t ci) {
     ServerClass &sc = s->cls[ci];
     std::unique_lock<std::mutex> lk(s->mu);
@@ -368,8 +371,33 @@ void complete_loop(fa_server *s, size_t ci) {
         if (sc.inflight.empty()) return;          // stopping, nothing left
         LaunchCtx *ctx = sc.inflight.front();
         lk.unlock();
-        const hipError_t e = hipEventSynchronize(ctx->done);
-        if (e != hipSuccess) {
+        hipError_t e = hipSuccess;
+        const fa::WaitEnd w = fa::wait_launch(
+            [&] {
+                e = hipEventQuery(ctx->done);
+                return e == hipSuccess ? 0 : (e == hipErrorNotReady ? 1 : 2);
+            },
+            [] { return std::chrono::duration<double>(Clock::now().time_since_epoch()).count(); },
+            [](uint64_t n) {
+                if (n < 2000) std::this_thread::yield();
+                else std::this_thread::sleep_for(std::chrono::microseconds(50));
+            },
+            kLaunchLimit);
+        if (w == fa::WaitEnd::TimedOut) {
+            // the launch's kernel has not finished: fail this launch's and
+            // every queued request of the class, keep its context busy (the
+            // kernel may still write its buffers) and stop waiting on the class
+            std::lock_guard<std::mutex> g(s->mu);
+            s->error = FA_EHIP;
+            s->error_msg = "class " + std::to_string(ci) + ": a launch has not completed after " +
+                           std::to_string((int)kLaunchLimit) + " s";
+            for (int id : ctx->clients) s->clients[id].state = 3;
+            for (int id : sc.queue) s->clients[id].state = 3;
+            sc.queue.clear();
+            s->cv_done.notify_all();
+            return;
+        }
+        if (w == fa::WaitEnd::Failed) {
             std::lock_guard<std::mutex> g(s->mu);
             s->error = FA_EHIP;
             s->error_msg = std::string("launch failed: ") + hipGetErrorString(e);

@@ -421,3 +421,79 @@ impl Drop for BatchAtariGpuEnv {
         unsafe { fa_vecenv_destroy(self.v) };
     }
 }
+
+// ---------------------------------------------------------------------------
+// Rendering console: pixel frames (Atari, src/atari.rs:226-378)
+// ---------------------------------------------------------------------------
+
+pub const FRAME_WIDTH: usize = 160;
+pub const FRAME_HEIGHT: usize = 192;
+pub const RENDER_STATE_BYTES: usize = 256;
+
+#[repr(C)]
+pub struct FaRender {
+    _private: [u8; 0],
+}
+
+extern "C" {
+    fn fa_render_create(rom: *const u8, rom_len: usize, n: usize, out: *mut *mut FaRender) -> c_int;
+    fn fa_render_destroy(r: *mut FaRender);
+    fn fa_render_reset(r: *mut FaRender, frames_out: *mut u8) -> c_int;
+    fn fa_render_step(r: *mut FaRender, actions: *const u8, n_actions: usize, k: u32, frames_out: *mut u8) -> c_int;
+    fn fa_render_download_states(r: *mut FaRender, out: *mut u8, n: usize) -> c_int;
+    fn fa_render_num_envs(r: *const FaRender) -> usize;
+}
+
+/// n rendering consoles of one ROM on the GPU: `Env::reset` / `Env::step`
+/// (src/atari.rs:330-357) for every env, frames as 160 x 192 NTSC palette
+/// indices per env (`Atari::framebuffer`, src/atari.rs:320-323).
+pub struct BatchAtariRenderGpu {
+    r: *mut FaRender,
+    n: usize,
+}
+
+unsafe impl Send for BatchAtariRenderGpu {}
+
+impl BatchAtariRenderGpu {
+    pub fn new(rom: Vec<u8>, n: usize) -> Result<Self, HipError> {
+        let mut r = std::ptr::null_mut();
+        check(unsafe { fa_render_create(rom.as_ptr(), rom.len(), n, &mut r) })?;
+        Ok(Self { r, n })
+    }
+
+    /// Atari::new + one run_frame per env; returns n frames.
+    pub fn reset(&mut self) -> Result<Vec<u8>, HipError> {
+        let mut frames = vec![0u8; self.n * FRAME_WIDTH * FRAME_HEIGHT];
+        check(unsafe { fa_render_reset(self.r, frames.as_mut_ptr()) })?;
+        Ok(frames)
+    }
+
+    /// k frames; actions laid out [f * n + i]; returns the frames after the last.
+    pub fn step_multi(&mut self, actions: &[u8], k: usize) -> Result<Vec<u8>, HipError> {
+        assert_eq!(actions.len(), k * self.n);
+        let mut frames = vec![0u8; self.n * FRAME_WIDTH * FRAME_HEIGHT];
+        check(unsafe { fa_render_step(self.r, actions.as_ptr(), actions.len(), k as u32, frames.as_mut_ptr()) })?;
+        Ok(frames)
+    }
+
+    pub fn step(&mut self, actions: &[u8]) -> Result<Vec<u8>, HipError> {
+        self.step_multi(actions, 1)
+    }
+
+    /// The 256-byte render states (AtariStateGpu layout, then the rendering TIA's fields).
+    pub fn download_states(&self) -> Result<Vec<u8>, HipError> {
+        let mut out = vec![0u8; self.n * RENDER_STATE_BYTES];
+        check(unsafe { fa_render_download_states(self.r, out.as_mut_ptr(), self.n) })?;
+        Ok(out)
+    }
+
+    pub fn num_envs(&self) -> usize {
+        unsafe { fa_render_num_envs(self.r) }
+    }
+}
+
+impl Drop for BatchAtariRenderGpu {
+    fn drop(&mut self) {
+        unsafe { fa_render_destroy(self.r) };
+    }
+}
