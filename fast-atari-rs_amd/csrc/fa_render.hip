@@ -112,7 +112,7 @@ namespace {
 using fa::fail;
 namespace rnd = fa::rnd;
 
-size_t render_lds(uint32_t rom_bytes, uint32_t bs) { return rnd::kTableBytes + rom_bytes + (size_t)bs * (128u + 16u); }
+size_t render_lds(uint32_t rom_bytes, uint32_t bs) { return rnd::kTableBytes + rom_bytes + (size_t)bs * rnd::kLaneBytes; }
 
 uint32_t render_block(const fa_render *r) { return r->block ? r->block : 64u; }
 

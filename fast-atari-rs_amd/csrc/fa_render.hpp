@@ -152,10 +152,30 @@ constexpr uint32_t rdesc_of(uint32_t code) {
     return P_ILLEGAL;
 }
 
-// LDS tables of one workgroup: micro-programs, then per opcode {rdesc, d0, d1}.
+// LDS tables of one workgroup: micro-programs, per opcode {rdesc, d0, d1},
+// then the collision table: for each set of objects on one pixel (bit 0 P0,
+// 1 P1, 2 M0, 3 M1, 4 BL, 5 PF) the latch bits it sets, as the two words
+// CXM0P..CXP1FB and CXM0FB..CXPPMM (one byte per register, D7 / D6).
 constexpr uint32_t kProgBytes = P_COUNT * kProgLen * 4;
 constexpr uint32_t kOpBytes = 256 * 16;
-constexpr uint32_t kTableBytes = kProgBytes + kOpBytes;
+constexpr uint32_t kCxTab = kProgBytes + kOpBytes;
+constexpr uint32_t kCxBytes = 64 * 8;
+constexpr uint32_t kTableBytes = kCxTab + kCxBytes;
+// LDS bytes per lane: RAM, the open 16-pixel group, and the six objects'
+// 160-pixel line masks (5 words each).
+constexpr uint32_t kMaskWords = 5;
+constexpr uint32_t kLaneBytes = 128 + 16 + 6 * kMaskWords * 4;
+
+// Collision latches set by one pixel holding the objects of `o`
+// (src/tia.rs:414-428, the reference's collision bit map).
+constexpr void collisions_of(uint32_t o, uint32_t &a, uint32_t &b) {
+    const uint32_t p0 = o & 1u, p1 = (o >> 1) & 1u, m0 = (o >> 2) & 1u, m1 = (o >> 3) & 1u;
+    const uint32_t bl = (o >> 4) & 1u, pf = (o >> 5) & 1u;
+    a = (m0 & p1) << 7 | (m0 & p0) << 6 | (m1 & p0) << 15 | (m1 & p1) << 14 | (p0 & pf) << 23 | (p0 & bl) << 22 |
+        (p1 & pf) << 31 | (p1 & bl) << 30;
+    b = (m0 & pf) << 7 | (m0 & bl) << 6 | (m1 & pf) << 15 | (m1 & bl) << 14 | (bl & pf) << 23 | (p0 & p1) << 31 |
+        (m0 & m1) << 30;
+}
 
 inline void build_tables(uint32_t *out) {
     for (uint32_t p = 0; p < P_COUNT; ++p) {
@@ -170,6 +190,8 @@ inline void build_tables(uint32_t *out) {
         ops[4 * c + 2] = (uint32_t)(d >> 32);
         ops[4 * c + 3] = 0;
     }
+    uint32_t *cx = out + kCxTab / 4;
+    for (uint32_t o = 0; o < 64; ++o) collisions_of(o, cx[2 * o], cx[2 * o + 1]);
 }
 
 }  // namespace rnd

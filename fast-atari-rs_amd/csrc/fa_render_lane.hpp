@@ -23,6 +23,7 @@ constexpr uint32_t kRState = FA_RENDER_STATE_BYTES;
 constexpr uint32_t kLoopBCap = 100000u;           // run_frame's loop B cap (src/atari.rs:296)
 constexpr uint32_t kLoopACap = 1000000u;          // loop A is unbounded in the reference (FA_ENV_VSYNC_HELD here)
 constexpr uint32_t kNoGroup = 0xFFFFFFFFu;
+constexpr uint32_t kNoWord = 0xFFFFFFFFu;
 
 struct RArgs {
     uint8_t *states;              // kRState bytes per env
@@ -66,39 +67,58 @@ __device__ __forceinline__ uint32_t hmove_pos(uint32_t pos, uint32_t hm) {
 // Copies after the first, as a mask over the offsets {16, 32, 64}, by NUSIZ
 // mode 0..7 (src/tia.rs:470-480 and 507-514: modes 5 and 7 draw one copy).
 __device__ __forceinline__ uint32_t copy_mask(uint32_t nusiz) { return (0x06043210u >> ((nusiz & 7u) * 4u)) & 7u; }
-// (o - off) mod 160 for 0 <= o < 160: x's offset past a copy `off` pixels to
-// the right of the first, given its offset o past the first.
-__device__ __forceinline__ uint32_t past(uint32_t o, uint32_t off) { return o >= off ? o - off : o + 160u - off; }
-__device__ __forceinline__ uint32_t offset_of(uint32_t x, uint32_t pos) {
-    const uint32_t o = x + 160u - pos;
-    return o >= 160u ? o - 160u : o;
+// Each bit of b8 spread over 2 / 4 bits (bit i -> bits 2i..2i+1 / 4i..4i+3).
+__device__ __forceinline__ uint32_t spread2(uint32_t b8) {
+    uint32_t x = b8 & 0xFFu;
+    x = (x | (x << 4)) & 0x0F0Fu;
+    x = (x | (x << 2)) & 0x3333u;
+    x = (x | (x << 1)) & 0x5555u;
+    return x * 3u;
 }
-
-// Player pixel (get_player_pixel, src/tia.rs:451-494); g is the drawn byte,
-// already reflected, leftmost pixel in bit 7.
-__device__ __forceinline__ uint32_t player_px(uint32_t x, uint32_t pos, uint32_t g, uint32_t nusiz) {
-    const uint32_t m = nusiz & 7u;
-    const uint32_t wsh = m == 5u ? 1u : (m == 7u ? 2u : 0u);
-    const uint32_t span = 8u << wsh;
-    const uint32_t cm = copy_mask(nusiz);
-    const uint32_t o0 = offset_of(x, pos);
-    const uint32_t o1 = past(o0, 16u), o2 = past(o0, 32u), o3 = past(o0, 64u);
-    uint32_t hit = o0 < span ? (g >> (7u - (o0 >> wsh))) & 1u : 0u;
-    hit |= ((cm & 1u) && o1 < 8u) ? (g >> (7u - o1)) & 1u : 0u;
-    hit |= ((cm & 2u) && o2 < 8u) ? (g >> (7u - o2)) & 1u : 0u;
-    hit |= ((cm & 4u) && o3 < 8u) ? (g >> (7u - o3)) & 1u : 0u;
-    return hit;
+__device__ __forceinline__ uint32_t spread4(uint32_t b8) {
+    uint32_t x = b8 & 0xFFu;
+    x = (x | (x << 12)) & 0x000F000Fu;
+    x = (x | (x << 6)) & 0x03030303u;
+    x = (x | (x << 3)) & 0x11111111u;
+    return x * 0xFu;
 }
-// Missile pixel (get_missile_pixel, src/tia.rs:497-522)
-__device__ __forceinline__ uint32_t missile_px(uint32_t x, uint32_t pos, uint32_t nusiz) {
-    const uint32_t w = 1u << ((nusiz >> 4) & 3u);
+// ORs a pattern (bit j = pixel s + j, at most 32 pixels) into a 160-pixel
+// line mask (m[w] bit i = pixel 32w + i), wrapping at pixel 160 as object
+// positions do (src/tia.rs:462-466).
+__device__ __forceinline__ void place(uint32_t (&m)[kMaskWords], uint32_t pat, uint32_t s) {
+    const uint32_t w = s >> 5, r = s & 31u;
+    const uint32_t lo = pat << r, hi = r ? (pat >> (32u - r)) : 0u;
+    const uint32_t w1 = w == kMaskWords - 1u ? 0u : w + 1u;
+#pragma unroll
+    for (uint32_t k = 0; k < kMaskWords; ++k) m[k] |= (k == w ? lo : 0u) | (k == w1 ? hi : 0u);
+}
+__device__ __forceinline__ uint32_t add160(uint32_t pos, uint32_t off) {
+    const uint32_t q = pos + off;
+    return q >= 160u ? q - 160u : q;
+}
+// Line mask of a player (get_player_pixel, src/tia.rs:451-494): byte g
+// (leftmost pixel in bit 7, mirrored when refp), 1 / 2 / 4 pixels a bit by
+// NUSIZ mode, copies 16 / 32 / 64 pixels right of the first.
+__device__ __forceinline__ void player_mask(uint32_t (&m)[kMaskWords], uint32_t pos, uint32_t g, uint32_t refp,
+                                            uint32_t nusiz) {
+    const uint32_t mode = nusiz & 7u;
+    const uint32_t b8 = refp ? (g & 0xFFu) : rev8(g);
+    const uint32_t pat = mode == 5u ? spread2(b8) : (mode == 7u ? spread4(b8) : b8);
     const uint32_t cm = copy_mask(nusiz);
-    const uint32_t o0 = offset_of(x, pos);
-    uint32_t hit = o0 < w;
-    hit |= (cm & 1u) && past(o0, 16u) < w;
-    hit |= (cm & 2u) && past(o0, 32u) < w;
-    hit |= (cm & 4u) && past(o0, 64u) < w;
-    return hit;
+    place(m, pat, pos);
+    if (cm & 1u) place(m, pat, add160(pos, 16u));
+    if (cm & 2u) place(m, pat, add160(pos, 32u));
+    if (cm & 4u) place(m, pat, add160(pos, 64u));
+}
+// Line mask of a missile (get_missile_pixel, src/tia.rs:497-522): NUSIZ bits
+// 4-5 give 1 / 2 / 4 / 8 pixels, copies as the player's.
+__device__ __forceinline__ void missile_mask(uint32_t (&m)[kMaskWords], uint32_t pos, uint32_t on, uint32_t nusiz) {
+    const uint32_t pat = on ? (1u << (1u << ((nusiz >> 4) & 3u))) - 1u : 0u;
+    const uint32_t cm = copy_mask(nusiz);
+    place(m, pat, pos);
+    if (cm & 1u) place(m, pat, add160(pos, 16u));
+    if (cm & 2u) place(m, pat, add160(pos, 32u));
+    if (cm & 4u) place(m, pat, add160(pos, 64u));
 }
 
 // One lane's console.
@@ -123,12 +143,19 @@ struct RLane {
     // mask, and the zero fill's next chunk (kChunks: none pending)
     uint32_t gkey, gmask, zq;
     // addressing
-    uint32_t ram0, grp0ff, stride4;   // LDS: this lane's RAM and pixel group, bytes between interleaved dwords
+    uint32_t ram0, grp0ff, msk0, stride4;   // LDS: this lane's RAM, pixel group and line masks
     uint32_t hot_lo, hot_n, rom0;
     uint8_t *frame;                   // this env's frame in HBM
+    // The six objects' line masks (P0, P1, M0, M1, BL, PF; LDS, rebuilt when
+    // a register they depend on is written) and, in registers, the words
+    // holding pixels 32 cw .. 32 cw + 31 of each (cw = kNoWord: not loaded).
+    uint32_t cw, c_p0, c_p1, c_m0, c_m1, c_bl, c_pf;
 
     __device__ __forceinline__ uint32_t ram_off(uint32_t i) const { return ram0 + (i >> 2) * stride4 + (i & 3u); }
     __device__ __forceinline__ uint32_t grp_off(uint32_t i) const { return grp0ff + (i >> 2) * stride4 + (i & 3u); }
+    __device__ __forceinline__ uint32_t msk_off(uint32_t obj, uint32_t w) const {
+        return msk0 + (obj * kMaskWords + w) * stride4;
+    }
 
     // the 20 playfield bits in display order: bit i lights pixels 4i..4i+3 of the left half
     __device__ __forceinline__ void update_pf20() {
@@ -160,6 +187,70 @@ struct RLane {
             reinterpret_cast<uint4 *>(frame)[zq] = make_uint4(0, 0, 0, 0);
             zq += 1u;
         }
+    }
+
+    // ---------------------------------------------------------- line masks
+    __device__ __forceinline__ void store_mask(uint32_t obj, const uint32_t (&m)[kMaskWords]) {
+#pragma unroll
+        for (uint32_t k = 0; k < kMaskWords; ++k) wr32(msk_off(obj, k), m[k]);
+    }
+    // Rebuilds the masks of the objects in `objs` (bit i: object i).
+    __device__ void rebuild(uint32_t objs) {
+        if (objs & 1u) {
+            uint32_t m[kMaskWords] = {0, 0, 0, 0, 0};
+            player_mask(m, pp0, pick(vdelp0, grp0o, grp0), refp0, nusiz0);
+            store_mask(0, m);
+        }
+        if (objs & 2u) {
+            uint32_t m[kMaskWords] = {0, 0, 0, 0, 0};
+            player_mask(m, pp1, pick(vdelp1, grp1o, grp1), refp1, nusiz1);
+            store_mask(1, m);
+        }
+        if (objs & 4u) {
+            uint32_t m[kMaskWords] = {0, 0, 0, 0, 0};
+            missile_mask(m, pm0, enam0 && !resmp0, nusiz0);
+            store_mask(2, m);
+        }
+        if (objs & 8u) {
+            uint32_t m[kMaskWords] = {0, 0, 0, 0, 0};
+            missile_mask(m, pm1, enam1 && !resmp1, nusiz1);
+            store_mask(3, m);
+        }
+        if (objs & 16u) {
+            // ball (get_ball_pixel, src/tia.rs:525-532): CTRLPF bits 4-5 give 1 / 2 / 4 / 8 pixels
+            uint32_t m[kMaskWords] = {0, 0, 0, 0, 0};
+            const uint32_t on = pick(vdelbl, enablo, enabl);
+            place(m, on ? (1u << (1u << ((ctrlpf >> 4) & 3u))) - 1u : 0u, pbl);
+            store_mask(4, m);
+        }
+        if (objs & 32u) {
+            // playfield (get_playfield_bit, src/tia.rs:432-448): each of the 20
+            // bits lights 4 pixels of the left half; the right half repeats
+            // it, or mirrors it when CTRLPF bit 0 is set
+            const uint32_t l0 = spread4(pf20), l1 = spread4(pf20 >> 8), l2 = spread4((pf20 >> 16) & 0xFu);
+            uint32_t r2, r3, r4;
+            if (ctrlpf & 1u) {
+                r2 = __builtin_bitreverse32(l2);
+                r3 = __builtin_bitreverse32(l1);
+                r4 = __builtin_bitreverse32(l0);
+            } else {
+                r2 = spread4(pf20 & 0xFu) << 16;
+                r3 = spread4(pf20 >> 4);
+                r4 = spread4(pf20 >> 12);
+            }
+            const uint32_t m[kMaskWords] = {l0, l1, (l2 & 0xFFFFu) | (r2 & 0xFFFF0000u), r3, r4};
+            store_mask(5, m);
+        }
+        cw = kNoWord;
+    }
+    __device__ __forceinline__ void load_words(uint32_t w) {
+        c_p0 = rd32(msk_off(0, w));
+        c_p1 = rd32(msk_off(1, w));
+        c_m0 = rd32(msk_off(2, w));
+        c_m1 = rd32(msk_off(3, w));
+        c_bl = rd32(msk_off(4, w));
+        c_pf = rd32(msk_off(5, w));
+        cw = w;
     }
 
     // ---------------------------------------------------------- TIA
@@ -207,15 +298,10 @@ struct RLane {
         const uint32_t vis = (line - vtop) & 0xFFFFu;
         const uint32_t px = (clock - 68u) & 0xFFFFu;
         if (clock >= 68u && px < kW && !blank && vis < kH) {
-            const uint32_t pfx = px < 80u ? (px >> 2) : ((ctrlpf & 1u) ? 39u - (px >> 2) : (px >> 2) - 20u);
-            const uint32_t pf = (pf20 >> pfx) & 1u;
-            const uint32_t g0 = pick(vdelp0, grp0o, grp0), g1 = pick(vdelp1, grp1o, grp1);
-            const uint32_t p0 = player_px(px, pp0, refp0 ? rev8(g0) : g0, nusiz0);
-            const uint32_t p1 = player_px(px, pp1, refp1 ? rev8(g1) : g1, nusiz1);
-            const uint32_t m0 = (enam0 && !resmp0) ? missile_px(px, pm0, nusiz0) : 0u;
-            const uint32_t m1 = (enam1 && !resmp1) ? missile_px(px, pm1, nusiz1) : 0u;
-            const uint32_t bon = pick(vdelbl, enablo, enabl);
-            const uint32_t bl = (bon && offset_of(px, pbl) < (1u << ((ctrlpf >> 4) & 3u))) ? 1u : 0u;
+            if ((px >> 5) != cw) load_words(px >> 5);
+            const uint32_t sh = px & 31u;
+            const uint32_t p0 = (c_p0 >> sh) & 1u, p1 = (c_p1 >> sh) & 1u, m0 = (c_m0 >> sh) & 1u;
+            const uint32_t m1 = (c_m1 >> sh) & 1u, bl = (c_bl >> sh) & 1u, pf = (c_pf >> sh) & 1u;
             const uint32_t pfc = pick(ctrlpf & 2u, pick(px < 80u, colup0, colup1), colupf);
             const uint32_t s0 = p0 | m0, s1 = p1 | m1, sf = pf | bl;
             uint32_t color;
@@ -229,10 +315,9 @@ struct RLane {
             }
             wr8(grp_off(px & 15u), color);
             gmask |= 1u << (px & 15u);
-            cxa |= (m0 & p1) << 7 | (m0 & p0) << 6 | (m1 & p0) << 15 | (m1 & p1) << 14 |
-                   (p0 & pf) << 23 | (p0 & bl) << 22 | (p1 & pf) << 31 | (p1 & bl) << 30;
-            cxb |= (m0 & pf) << 7 | (m0 & bl) << 6 | (m1 & pf) << 15 | (m1 & bl) << 14 |
-                   (bl & pf) << 23 | (p0 & p1) << 31 | (m0 & m1) << 30;
+            const uint32_t o6 = p0 | (p1 << 1) | (m0 << 2) | (m1 << 3) | (bl << 4) | (pf << 5);
+            cxa |= rd32(kCxTab + o6 * 8u);
+            cxb |= rd32(kCxTab + o6 * 8u + 4u);
         }
         clock += 1u;
         if (clock >= 228u) line_end();
@@ -253,6 +338,7 @@ struct RLane {
 
     // Tia::write, src/tia.rs:593-673
     __device__ __forceinline__ void tia_write(uint32_t a13, uint32_t v) {
+        uint32_t objs = 0;               // line masks to rebuild
         switch (a13 & 0x3Fu) {
         case 0x00:
             if (!(vsync & 2u) && (v & 2u)) {
@@ -279,38 +365,38 @@ struct RLane {
             vblank = v;
             break;
         case 0x02: wsync = 1; break;
-        case 0x04: nusiz0 = v; break;
-        case 0x05: nusiz1 = v; break;
+        case 0x04: nusiz0 = v; objs = 1u | 4u; break;
+        case 0x05: nusiz1 = v; objs = 2u | 8u; break;
         case 0x06: colup0 = v; break;
         case 0x07: colup1 = v; break;
         case 0x08: colupf = v; break;
         case 0x09: colubk = v; break;
-        case 0x0A: ctrlpf = v; break;
-        case 0x0B: refp0 = (v >> 3) & 1u; break;
-        case 0x0C: refp1 = (v >> 3) & 1u; break;
-        case 0x0D: pf0 = v; update_pf20(); break;
-        case 0x0E: pf1 = v; update_pf20(); break;
-        case 0x0F: pf2 = v; update_pf20(); break;
-        case 0x10: pp0 = resp_player(clock); break;
-        case 0x11: pp1 = resp_player(clock); break;
-        case 0x12: pm0 = resp_simple(clock); break;
-        case 0x13: pm1 = resp_simple(clock); break;
-        case 0x14: pbl = resp_simple(clock); break;
-        case 0x1B: grp0o = grp0; grp0 = v; break;
-        case 0x1C: grp1o = grp1; grp1 = v; enablo = enabl; break;
-        case 0x1D: enam0 = (v >> 1) & 1u; break;
-        case 0x1E: enam1 = (v >> 1) & 1u; break;
-        case 0x1F: enabl = (v >> 1) & 1u; break;
+        case 0x0A: ctrlpf = v; objs = 16u | 32u; break;
+        case 0x0B: refp0 = (v >> 3) & 1u; objs = 1u; break;
+        case 0x0C: refp1 = (v >> 3) & 1u; objs = 2u; break;
+        case 0x0D: pf0 = v; update_pf20(); objs = 32u; break;
+        case 0x0E: pf1 = v; update_pf20(); objs = 32u; break;
+        case 0x0F: pf2 = v; update_pf20(); objs = 32u; break;
+        case 0x10: pp0 = resp_player(clock); objs = 1u; break;
+        case 0x11: pp1 = resp_player(clock); objs = 2u; break;
+        case 0x12: pm0 = resp_simple(clock); objs = 4u; break;
+        case 0x13: pm1 = resp_simple(clock); objs = 8u; break;
+        case 0x14: pbl = resp_simple(clock); objs = 16u; break;
+        case 0x1B: grp0o = grp0; grp0 = v; objs = 1u; break;
+        case 0x1C: grp1o = grp1; grp1 = v; enablo = enabl; objs = 2u | 16u; break;
+        case 0x1D: enam0 = (v >> 1) & 1u; objs = 4u; break;
+        case 0x1E: enam1 = (v >> 1) & 1u; objs = 8u; break;
+        case 0x1F: enabl = (v >> 1) & 1u; objs = 16u; break;
         case 0x20: hp0 = v; break;
         case 0x21: hp1 = v; break;
         case 0x22: hmm0 = v; break;
         case 0x23: hmm1 = v; break;
         case 0x24: hbl = v; break;
-        case 0x25: vdelp0 = v & 1u; break;
-        case 0x26: vdelp1 = v & 1u; break;
-        case 0x27: vdelbl = v & 1u; break;
-        case 0x28: resmp0 = (v >> 1) & 1u; break;
-        case 0x29: resmp1 = (v >> 1) & 1u; break;
+        case 0x25: vdelp0 = v & 1u; objs = 1u; break;
+        case 0x26: vdelp1 = v & 1u; objs = 2u; break;
+        case 0x27: vdelbl = v & 1u; objs = 16u; break;
+        case 0x28: resmp0 = (v >> 1) & 1u; objs = 4u; break;
+        case 0x29: resmp1 = (v >> 1) & 1u; objs = 8u; break;
         case 0x2A:
             pp0 = hmove_pos(pp0, hp0);
             pp1 = hmove_pos(pp1, hp1);
@@ -318,11 +404,13 @@ struct RLane {
             pm1 = hmove_pos(pm1, hmm1);
             pbl = hmove_pos(pbl, hbl);
             hmove = 1;
+            objs = 31u;
             break;
         case 0x2B: hp0 = hp1 = hmm0 = hmm1 = hbl = 0; break;
         case 0x2C: cxa = cxb = 0; break;
         default: break;           // RSYNC, audio, unused
         }
+        if (objs) rebuild(objs);
     }
 
     // ---------------------------------------------------------- RIOT
@@ -651,16 +739,19 @@ __device__ __forceinline__ void render_lane(const RArgs &g, uint32_t bs, uint32_
     const uint32_t rom0 = kTableBytes;
     const uint32_t ram_base = rom0 + g.rom_bytes;
     const uint32_t grp_base = ram_base + 128u * bs;
+    const uint32_t msk_base = grp_base + 16u * bs;
     RLane e;
     e.stride4 = 4u * bs;
     e.ram0 = ram_base + 4u * tid;
     e.grp0ff = grp_base + 4u * tid;
+    e.msk0 = msk_base + 4u * tid;
     e.hot_lo = g.hot_lo;
     e.hot_n = g.hot_n;
     e.rom0 = rom0;
     e.frame = g.fb + (size_t)idx * kFrameBytes;
     uint8_t *st_ptr = g.states + (size_t)idx * kRState;
     load_lane(e, st_ptr);
+    e.rebuild(63u);
     const uint32_t scheme = st_ptr[218];
     e.busy = 0;
     e.rx = e.d0 = e.d1 = e.t = e.cyc = e.opnd = e.lo = e.hi = e.dat = e.res = 0;

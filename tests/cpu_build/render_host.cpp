@@ -32,7 +32,7 @@ extern "C" uint32_t rh_table_bytes(void) { return kTableBytes; }
 extern "C" int rh_launch(const uint8_t *rom_img, uint32_t rom_bytes, uint32_t hot_lo, uint32_t hot_n, uint8_t *states,
                          uint64_t *cpu, uint64_t *fcyc, uint64_t *instr, uint32_t *status, const uint8_t *actions,
                          uint8_t *fb, uint32_t n, uint32_t k, uint32_t bs) {
-    if (kTableBytes + rom_bytes + (size_t)bs * 144u > sizeof(r_lds)) return -1;
+    if (kTableBytes + rom_bytes + (size_t)bs * kLaneBytes > sizeof(r_lds)) return -1;
     static uint32_t tables[kTableBytes / 4];
     build_tables(tables);
     RArgs g{};
