@@ -152,29 +152,26 @@ constexpr uint32_t rdesc_of(uint32_t code) {
     return P_ILLEGAL;
 }
 
-// LDS tables of one workgroup: micro-programs, per opcode {rdesc, d0, d1},
-// then the collision table: for each set of objects on one pixel (bit 0 P0,
-// 1 P1, 2 M0, 3 M1, 4 BL, 5 PF) the latch bits it sets, as the two words
-// CXM0P..CXP1FB and CXM0FB..CXPPMM (one byte per register, D7 / D6).
+// LDS tables of one workgroup: micro-programs, then per opcode {rdesc, d0,
+// g, h}: for compute-class opcodes g / h are the one-hot selection words of
+// the per-lane compute stage (fa_decode.hpp d2 / d3); for branches h holds
+// d1's tested flag mask and taken-when-set bit (D1_FMASK_SHIFT,
+// D1_FVAL_SHIFT).
 constexpr uint32_t kProgBytes = P_COUNT * kProgLen * 4;
 constexpr uint32_t kOpBytes = 256 * 16;
-constexpr uint32_t kCxTab = kProgBytes + kOpBytes;
-constexpr uint32_t kCxBytes = 64 * 8;
-constexpr uint32_t kTableBytes = kCxTab + kCxBytes;
+constexpr uint32_t kTableBytes = kProgBytes + kOpBytes;
 // LDS bytes per lane: RAM, the open 16-pixel group, and the six objects'
 // 160-pixel line masks (5 words each).
 constexpr uint32_t kMaskWords = 5;
 constexpr uint32_t kLaneBytes = 128 + 16 + 6 * kMaskWords * 4;
 
-// Collision latches set by one pixel holding the objects of `o`
-// (src/tia.rs:414-428, the reference's collision bit map).
-constexpr void collisions_of(uint32_t o, uint32_t &a, uint32_t &b) {
-    const uint32_t p0 = o & 1u, p1 = (o >> 1) & 1u, m0 = (o >> 2) & 1u, m1 = (o >> 3) & 1u;
-    const uint32_t bl = (o >> 4) & 1u, pf = (o >> 5) & 1u;
-    a = (m0 & p1) << 7 | (m0 & p0) << 6 | (m1 & p0) << 15 | (m1 & p1) << 14 | (p0 & pf) << 23 | (p0 & bl) << 22 |
-        (p1 & pf) << 31 | (p1 & bl) << 30;
-    b = (m0 & pf) << 7 | (m0 & bl) << 6 | (m1 & pf) << 15 | (m1 & bl) << 14 | (bl & pf) << 23 | (p0 & p1) << 31 |
-        (m0 & m1) << 30;
+// The sets of objects on one pixel (bit 0 P0, 1 P1, 2 M0, 3 M1, 4 BL, 5 PF)
+// in which objects i and j overlap, as a 64-bit mask over the 64 sets.
+constexpr uint64_t overlap_sets(uint32_t i, uint32_t j) {
+    uint64_t m = 0;
+    for (uint32_t o = 0; o < 64; ++o)
+        if (((o >> i) & 1u) && ((o >> j) & 1u)) m |= 1ull << o;
+    return m;
 }
 
 inline void build_tables(uint32_t *out) {
@@ -185,13 +182,15 @@ inline void build_tables(uint32_t *out) {
     uint32_t *ops = out + kProgBytes / 4;
     for (uint32_t c = 0; c < 256; ++c) {
         const uint64_t d = descriptor_of(c);
+        const uint32_t d0 = (uint32_t)d, d1 = (uint32_t)(d >> 32);
+        uint32_t g = 0, h = 0;
+        describe_lanes(d0, d1, g, h);
+        if ((d0 & 3u) == CLS_BRANCH) h = d1 & 0xFFFF0000u;
         ops[4 * c] = rdesc_of(c);
-        ops[4 * c + 1] = (uint32_t)d;
-        ops[4 * c + 2] = (uint32_t)(d >> 32);
-        ops[4 * c + 3] = 0;
+        ops[4 * c + 1] = d0;
+        ops[4 * c + 2] = g;
+        ops[4 * c + 3] = h;
     }
-    uint32_t *cx = out + kCxTab / 4;
-    for (uint32_t o = 0; o < 64; ++o) collisions_of(o, cx[2 * o], cx[2 * o + 1]);
 }
 
 }  // namespace rnd

@@ -46,6 +46,8 @@ __device__ __forceinline__ uint32_t rev8(uint32_t v) { return __builtin_bitrever
 // members of the lane's state is a choice of values, never of addresses,
 // and the state stays in registers.
 __device__ __forceinline__ uint32_t pick(bool c, uint32_t t, uint32_t f) { return c ? t : f; }
+// All ones when bit k of w is set, else zero.
+__device__ __forceinline__ uint32_t lbm(uint32_t w, uint32_t k) { return (uint32_t)((int32_t)(w << (31u - k)) >> 31); }
 
 // RESMx / RESBL: (clock - 68 + 5) mod 160 (calc_resp_position_simple, src/tia.rs:548-550)
 __device__ __forceinline__ uint32_t resp_simple(uint32_t clock) {
@@ -125,7 +127,7 @@ __device__ __forceinline__ void missile_mask(uint32_t (&m)[kMaskWords], uint32_t
 struct RLane {
     // CPU and the instruction in progress
     uint32_t a, x, y, sp, p, pc;
-    uint32_t busy, rx, d0, d1, t, cyc, opnd, lo, hi, dat, res;
+    uint32_t busy, rx, d0, dg, dh, t, cyc, opnd, lo, hi, dat, res;
     // run_frame's loop: in loop A (VSYNC held at the frame's start), cycles so far
     uint32_t in_a, fcyc;
     // TIA
@@ -135,6 +137,7 @@ struct RLane {
     uint32_t enam0, enam1, enabl, enablo, resmp0, resmp1;
     uint32_t pp0, pp1, pm0, pm1, pbl, hp0, hp1, hmm0, hmm1, hbl;
     uint32_t cxa, cxb;            // CXM0P, CXM1P, CXP0FB, CXP1FB / CXM0FB, CXM1FB, CXBLPF, CXPPMM, a byte each
+    uint64_t seen;                // object sets drawn since the latches were last brought up to date
     uint32_t inpt4, inpt5, pad0, pad1, pcount, pdump;
     uint32_t hmove, vtop, vbtop, ptop, pframes, l4, l5, len;
     // RIOT and cartridge
@@ -315,18 +318,33 @@ struct RLane {
             }
             wr8(grp_off(px & 15u), color);
             gmask |= 1u << (px & 15u);
-            const uint32_t o6 = p0 | (p1 << 1) | (m0 << 2) | (m1 << 3) | (bl << 4) | (pf << 5);
-            cxa |= rd32(kCxTab + o6 * 8u);
-            cxb |= rd32(kCxTab + o6 * 8u + 4u);
+            seen |= 1ull << (p0 | (p1 << 1) | (m0 << 2) | (m1 << 3) | (bl << 4) | (pf << 5));
         }
         clock += 1u;
         if (clock >= 228u) line_end();
     }
 
+    // The collision latches (src/tia.rs:414-428) from the object sets drawn
+    // since the last fold: a latch bit is set when a drawn set holds both of
+    // its objects.
+    __device__ __forceinline__ void fold_collisions() {
+        constexpr uint32_t P0 = 0, P1 = 1, M0 = 2, M1 = 3, BL = 4, PF = 5;
+        const uint64_t sn = seen;
+        auto hit = [sn](uint64_t sets, uint32_t bit) { return (sn & sets) ? (1u << bit) : 0u; };
+        cxa |= hit(overlap_sets(M0, P1), 7) | hit(overlap_sets(M0, P0), 6) | hit(overlap_sets(M1, P0), 15) |
+               hit(overlap_sets(M1, P1), 14) | hit(overlap_sets(P0, PF), 23) | hit(overlap_sets(P0, BL), 22) |
+               hit(overlap_sets(P1, PF), 31) | hit(overlap_sets(P1, BL), 30);
+        cxb |= hit(overlap_sets(M0, PF), 7) | hit(overlap_sets(M0, BL), 6) | hit(overlap_sets(M1, PF), 15) |
+               hit(overlap_sets(M1, BL), 14) | hit(overlap_sets(BL, PF), 23) | hit(overlap_sets(P0, P1), 31) |
+               hit(overlap_sets(M0, M1), 30);
+        seen = 0;
+    }
+
     // Tia::read, src/tia.rs:557-590
-    __device__ __forceinline__ uint32_t tia_read(uint32_t a13) const {
+    __device__ __forceinline__ uint32_t tia_read(uint32_t a13) {
         const uint32_t r = a13 & 0x0Fu;
         uint32_t v = 0;
+        if (r < 8u && seen) fold_collisions();
         if (r < 8u) v = (pick(r < 4u, cxa, cxb) >> ((r & 3u) * 8u)) & 0xFFu;
         else if (r == 8u) v = pcount >= pad0 ? 0x80u : 0u;
         else if (r == 9u) v = pcount >= pad1 ? 0x80u : 0u;
@@ -407,7 +425,7 @@ struct RLane {
             objs = 31u;
             break;
         case 0x2B: hp0 = hp1 = hmm0 = hmm1 = hbl = 0; break;
-        case 0x2C: cxa = cxb = 0; break;
+        case 0x2C: cxa = cxb = 0; seen = 0; break;
         default: break;           // RSYNC, audio, unused
         }
         if (objs) rebuild(objs);
@@ -520,77 +538,52 @@ struct RLane {
     }
     __device__ __forceinline__ void set_zn(uint32_t v) { p = (p & ~0x82u) | (v & 0x80u) | (v == 0u ? 0x02u : 0u); }
 
-    // The register / flag effect of an instruction on operand v; a memory
-    // read-modify-write leaves its result in `res` for the final write.
-    __device__ __forceinline__ void alu(uint32_t v) {
+    // The register / flag effect of the instruction (src/cpu.rs:245-1146) on
+    // its operand: the data byte read (memory or immediate) or the source
+    // register.  Straight-line code steered by the opcode's one-hot words
+    // (g: result and register selection, h: flag masks), so lanes at
+    // different opcodes share it; a memory read-modify-write's result goes to
+    // `res` for the final write.
+    __device__ __forceinline__ void alu() {
         const uint32_t spec = (rx >> RX_SPEC_SHIFT) & 7u;
-        if (spec == SX_PLA) {
-            a = v;
-            set_zn(v);
-            return;
-        }
-        if (spec == SX_PLP) {
-            p = v & ~0x30u;
-            return;
-        }
-        const uint32_t kind = (d0 >> D_KIND_SHIFT) & 7u;
-        const uint32_t s = reg((d1 >> D1_SRC_SHIFT) & 7u);
-        const uint32_t dstr = (d1 >> D1_DST_SHIFT) & 7u;
-        switch (kind) {
-        case K_LOGIC: {
-            const uint32_t r = (d0 & D_M_LD) ? v : ((d0 & D_M_XOR) ? ((d0 & D_M_AND) ? (a | v) : (a ^ v)) : (a & v));
-            set_reg(dstr, r);
-            set_zn(r);
-            break;
-        }
-        case K_ARITH: {
-            const uint32_t a0 = a, c0 = p & 1u;
-            const uint32_t vv = (d0 & D_INV) ? (v ^ 0xFFu) : v;
-            const uint32_t sum = s + vv + ((d0 & D_CIN1) ? 1u : c0);
-            const uint32_t r = sum & 0xFFu;
-            p = (p & ~1u) | ((sum >> 8) & 1u);
-            if (d0 & D_SETV) p = (p & ~0x40u) | (((~(s ^ vv) & (s ^ r)) & 0x80u) >> 1);
-            set_zn(r);
-            if (!(d0 & D_KEEP)) set_reg(dstr, r);
-            if ((d0 & D_DEC) && (p & 0x08u)) decimal((d0 & D_INV) != 0u, a0, v, c0);
-            break;
-        }
-        case K_BIT:
-            p = (p & ~0xC2u) | (v & 0xC0u) | ((a & v) == 0u ? 0x02u : 0u);
-            break;
-        case K_RMW: {
-            const uint32_t rop = (d1 >> D1_RMW_SHIFT) & 7u;
-            const bool mem = (d0 & D_MEM) != 0u;
-            const uint32_t in = pick(mem, v, a);
-            uint32_t r;
-            if (rop < 2u) {
-                r = ((in << 1) | (rop == 1u ? (p & 1u) : 0u)) & 0xFFu;
-                p = (p & ~1u) | (in >> 7);
-            } else if (rop < 4u) {
-                r = (in >> 1) | (rop == 3u ? ((p & 1u) << 7) : 0u);
-                p = (p & ~1u) | (in & 1u);
+        if (spec != SX_NONE) {           // PLA / PLP
+            if (spec == SX_PLA) {
+                a = dat;
+                set_zn(dat);
             } else {
-                r = (in + (rop == 5u ? 1u : 0xFFu)) & 0xFFu;
+                p = dat & ~0x30u;
             }
-            set_zn(r);
-            if (mem) res = r;
-            else a = r;
-            break;
+            return;
         }
-        case K_MOVE: {
-            const uint32_t dl = (d1 >> D1_DELTA_SHIFT) & 3u;
-            const uint32_t r = (s + (dl == 1u ? 1u : (dl == 2u ? 0xFFu : 0u))) & 0xFFu;
-            set_reg(dstr, r);
-            if (d0 & D_SETNZ) set_zn(r);
-            break;
-        }
-        case K_FLAGS: {
-            const uint32_t fm = (d1 >> D1_FMASK_SHIFT) & 0xFFu;
-            p = (p & ~fm) | (d1 >> D1_FVAL_SHIFT);
-            break;
-        }
-        default: break;           // K_STORE, K_NOP
-        }
+        const uint32_t g = dg, h = dh;
+        const uint32_t s = (a & lbm(g, G_SRC)) | (x & lbm(g, G_SRC + 1u)) | (y & lbm(g, G_SRC + 2u)) |
+                           (sp & lbm(g, G_SRC + 3u));
+        const uint32_t v = pick((d0 & (D_IMM | D_RD)) != 0u, dat, s);
+        const uint32_t p0 = p, a0 = a;
+        const uint32_t lg = (a & v & lbm(h, H_M_AND)) | ((a ^ v) & lbm(h, H_M_XOR)) | (v & lbm(h, H_M_LD));
+        const uint32_t vv = v ^ (0xFFu & lbm(d0, 25u));               // D_INV
+        const uint32_t cin = p & 1u;
+        const uint32_t sum = s + vv + (cin | ((d0 >> 26) & 1u));      // D_CIN1
+        const uint32_t ar = sum & 0xFFu;
+        const uint32_t shl = ((v << 1) | (cin & lbm(g, G_CIN_SHL))) & 0xFFu;
+        const uint32_t shr = (v >> 1) | ((cin << 7) & lbm(g, G_CIN_SHR));
+        const uint32_t inc = (v + (g >> G_DELTA)) & 0xFFu;
+        const uint32_t r = (lg & lbm(g, G_R_LOG)) | (ar & lbm(g, G_R_ARI)) | (shl & lbm(g, G_R_SHL)) |
+                           (shr & lbm(g, G_R_SHR)) | (inc & lbm(g, G_R_INC)) | (s & lbm(g, G_R_SRC));
+        // N, V, Z, C candidates, then the opcode's clear / take / set masks
+        const uint32_t mbit = lbm(g, G_BIT);
+        const uint32_t nv = (v & mbit) | (r & ~mbit);
+        const uint32_t vf = (((~(s ^ vv) & (s ^ ar)) >> 1) & 0x40u & ~mbit) | (v & 0x40u & mbit);
+        const uint32_t cf = (((sum >> 8) & lbm(g, G_C_SUM)) | ((v >> 7) & lbm(g, G_C_B7)) | (v & lbm(g, G_C_B0))) & 1u;
+        const uint32_t cand = (nv & 0x80u) | vf | (r == 0u ? 0x02u : 0u) | cf;
+        p = (p & ~(h & 0xFFu)) | (cand & (h >> H_TAKE)) | ((h >> H_SET) & 0xFFu);
+        const uint32_t ma = lbm(g, G_DST), mx = lbm(g, G_DST + 1u), my = lbm(g, G_DST + 2u), ms = lbm(g, G_DST + 3u);
+        a = (r & ma) | (a & ~ma);
+        x = (r & mx) | (x & ~mx);
+        y = (r & my) | (y & ~my);
+        sp = (r & ms) | (sp & ~ms);
+        res = r;
+        if ((d0 & D_DEC) && (p0 & 0x08u)) decimal((d0 & D_INV) != 0u, a0, v, cin);
     }
 
     __device__ __forceinline__ uint32_t idx() const { return pick(rx & RX_IDXY, y, x); }
@@ -598,8 +591,8 @@ struct RLane {
     __device__ __forceinline__ uint32_t ea() const { return (base() + idx()) & 0xFFFFu; }
     __device__ __forceinline__ bool cross() const { return ((base() ^ ea()) & 0xFF00u) != 0u; }
     __device__ __forceinline__ bool taken() const {
-        const uint32_t fm = (d1 >> D1_FMASK_SHIFT) & 0xFFu;
-        return ((p & fm) != 0u) == ((d1 >> D1_FVAL_SHIFT) != 0u);
+        const uint32_t fm = (dh >> D1_FMASK_SHIFT) & 0xFFu;
+        return ((p & fm) != 0u) == ((dh >> D1_FVAL_SHIFT) != 0u);
     }
     // branch target from the PC after the operand (src/cpu.rs:925-940)
     __device__ __forceinline__ uint32_t target() const { return (pc + opnd - ((opnd & 0x80u) << 1)) & 0xFFFFu; }
@@ -754,7 +747,8 @@ __device__ __forceinline__ void render_lane(const RArgs &g, uint32_t bs, uint32_
     e.rebuild(63u);
     const uint32_t scheme = st_ptr[218];
     e.busy = 0;
-    e.rx = e.d0 = e.d1 = e.t = e.cyc = e.opnd = e.lo = e.hi = e.dat = e.res = 0;
+    e.rx = e.d0 = e.dg = e.dh = e.t = e.cyc = e.opnd = e.lo = e.hi = e.dat = e.res = 0;
+    e.seen = 0;
     e.gkey = kNoGroup;
     e.gmask = 0;
     e.zq = kChunks;
@@ -816,7 +810,8 @@ __device__ __forceinline__ void render_lane(const RArgs &g, uint32_t bs, uint32_
                     const uint4 dsc = *reinterpret_cast<const uint4 *>(r_lds + kProgBytes + v * 16u);
                     e.rx = dsc.x;
                     e.d0 = dsc.y;
-                    e.d1 = dsc.z;
+                    e.dg = dsc.z;
+                    e.dh = dsc.w;
                     if ((e.rx & 31u) == P_ILLEGAL) {
                         st = FA_ENV_ILLEGAL_OPCODE;   // the reference panics (src/cpu.rs:1139-1141)
                         frame_end = true;
@@ -841,7 +836,7 @@ __device__ __forceinline__ void render_lane(const RArgs &g, uint32_t bs, uint32_
                 if (u & U_PCINC) e.pc = (e.pc + 1u) & 0xFFFFu;
                 if (u & U_SPDEC) e.sp = (e.sp - 1u) & 0xFFu;
                 if (u & U_SETI) e.p |= 0x04u;
-                if (u & U_ALU) e.alu(e.dat);
+                if (u & U_ALU) e.alu();
                 const bool last = (u & U_LAST) || ((u & U_LAST_NT) && !e.taken()) ||
                                   ((u & U_LAST_NC) && ((e.pc ^ e.target()) & 0xFF00u) == 0u);
                 if (last) {
@@ -893,6 +888,7 @@ __device__ __forceinline__ void render_lane(const RArgs &g, uint32_t bs, uint32_
         if (g.actions) set_action(e, g.actions[(size_t)(g.k - 1u) * g.stride + idx]);
     }
     while (e.zq < kChunks) e.zero_step();
+    e.fold_collisions();
     e.flush_group();
     store_lane(e, st_ptr, scheme);
     g.cpu_cycles[idx] = cpu_cyc;
