@@ -84,6 +84,11 @@ enum : uint32_t {
     H_TAKE = 8,             // 8 bits: status bits taken from the computed N, V, Z, C
     H_SET = 16,             // 8 bits: status bits forced to 1 (SEC, SED, SEI)
     H_M_AND = 24, H_M_XOR = 25, H_M_LD = 26,   // logic masks (d0's D_M_*, BIT: AND)
+    H_BR = 27,              // conditional branch: target = PC + offset (d1: flag mask, taken value)
+    H_JMP = 28,             // JMP absolute: PC = the operand
+    H_PUSH = 29,            // PHA / PHP: the source written at $0100 + SP, SP - 1
+    H_PULL = 30,            // PLA / PLP: SP + 1, the result read from $0100 + SP
+    H_PREG = 31,            // PHP / PLP: the status register is the source / destination
 };
 constexpr uint32_t kDescWords = 4;                  // words per opcode
 constexpr uint32_t kDescBytes = 256u * kDescWords * 4u;
@@ -92,6 +97,22 @@ constexpr uint32_t kDescBytes = 256u * kDescWords * 4u;
 constexpr void describe_lanes(uint32_t d0, uint32_t d1, uint32_t &d2, uint32_t &d3) {
     d2 = 0;
     d3 = 0;
+    constexpr uint32_t NZ = 0x82;
+    if ((d0 & 3u) == CLS_BRANCH) { d3 = 1u << H_BR; return; }
+    if ((d0 & 3u) == CLS_SPECIAL) {
+        switch ((d1 >> D1_SPEC_SHIFT) & 15u) {
+        case SP_JMP: d3 = 1u << H_JMP; break;
+        case SP_PHA: d2 = (1u << G_SRC) | (1u << G_R_SRC); d3 = 1u << H_PUSH; break;
+        case SP_PHP: d2 = 1u << G_R_SRC; d3 = (1u << H_PUSH) | (1u << H_PREG); break;
+        case SP_PLA:
+            d2 = (1u << (G_DST + R_A)) | (1u << G_R_LOG);
+            d3 = (1u << H_PULL) | (1u << H_M_LD) | (NZ << H_CLR) | (NZ << H_TAKE);
+            break;
+        case SP_PLP: d2 = 1u << G_R_LOG; d3 = (1u << H_PULL) | (1u << H_PREG) | (1u << H_M_LD) | (0xFFu << H_CLR); break;
+        default: break;
+        }
+        return;
+    }
     if ((d0 & 3u) != CLS_COMPUTE) return;
     const uint32_t kind = (d0 >> D_KIND_SHIFT) & 7u;
     const uint32_t src = (d1 >> D1_SRC_SHIFT) & 7u, dst = (d1 >> D1_DST_SHIFT) & 7u;

@@ -12,8 +12,9 @@
 // Instructions run through one shared pipeline (fetch, descriptor decode,
 // effective address, data read, compute, write-back) steered by the per-opcode
 // descriptors of fa_decode.hpp with predicated selects, so lanes executing
-// different opcodes stay on the same code path; only TIA / RIOT register
-// accesses, decimal-mode arithmetic and stack / jump instructions branch.
+// different opcodes stay on the same code path (branches, JMP and the
+// push / pull instructions included); only TIA / RIOT register writes, RIOT
+// reads, decimal-mode arithmetic and JSR / RTS / RTI / BRK / JMP (ind) branch.
 //
 // LDS layout of one workgroup (byte offsets into fa_lds):
 //   [0, 4096)                     opcode descriptors (256 x 16 bytes)
@@ -36,6 +37,9 @@ namespace fa {
 
 constexpr uint32_t DESC0 = 0;        // LDS offset of the descriptor table
 constexpr uint32_t ROM0 = kDescBytes; // LDS offset of the ROM image
+// A byte that always reads 0: the second word of an undocumented opcode's
+// descriptor ($02).
+constexpr uint32_t ZERO_BYTE = DESC0 + 0x02u * 16u + 4u;
 
 // Byte offsets inside the 222-byte AtariStateGpu (src/cuda_env.rs:12-38).
 enum : uint32_t {
@@ -467,7 +471,7 @@ struct Emu {
     // and come straight from LDS; elsewhere they go through the bus in the
     // reference's order (opcode, then operands or the phantom read of PC).
     // JSR's high operand byte is read here for its side effects (an INTIM
-    // read, a hotspot) and read again after the two pushes (execute()).
+    // read, a hotspot) and read again after the two pushes (execute_slow()).
     __device__ __forceinline__ void fetch(uint32_t &b1, uint32_t &b2, uint4 &d, uint32_t &op) {
         if ((pc & 0x1000u) && (pc & 0xFFFu) < 0xFF0u) {
             const uint32_t o = ROM0 + (bank << 12) + (pc & 0xFFFu);
@@ -519,52 +523,87 @@ struct Emu {
         p = pick(dec & !sub, np, p);
     }
 
-    // Compute-class instruction (loads, stores, ALU, read-modify-write,
-    // transfers, flag instructions) on the per-lane path: every lane runs the
-    // same straight-line code, steered by the one-hot words d2 / d3, and only
-    // TIA / RIOT accesses, zero-page pointers in the TIA and decimal mode
-    // branch, each on a ballot of the lanes that need it.  Bus accesses are
-    // those of src/cpu.rs in the reference's order; the phantom write of a
-    // read-modify-write is made only where it can be seen (TIA / RIOT), since
-    // on RAM the final write replaces it and on ROM both writes select the
-    // same bank.
-    __device__ __forceinline__ uint32_t compute_lanes(uint32_t b1, uint32_t b2, uint32_t d0, uint32_t g, uint32_t h) {
+    // TIA read registers from the lane's image: register r4 (addr & 15) is
+    // read from the byte at S_CX + tia_lut(r4) and fixed by tia_fix.
+    // CXxx (0-7) at S_CX + r, the paddle values (8, 9) at S_PAD0 = S_CX + 10,
+    // INPT4 / INPT5 (12, 13) at S_INPT4 = S_CX + 8; 10, 11, 14, 15 read
+    // constants (the byte is ignored).
+    __device__ __forceinline__ static uint32_t tia_lut(uint32_t r4) {
+        return pick(r4 < 8u, r4, __builtin_amdgcn_ubfe(0x009800BAu, (r4 & 7u) << 2, 4u));
+    }
+    // 8, 9: 0x80 once the paddle counter has reached the value; 10, 11: 0x80;
+    // 12, 13: 0x80 while the input latch is set; 14, 15: 0 (src/headless_tia.rs:205-223)
+    __device__ __forceinline__ uint32_t tia_fix(uint32_t r4, uint32_t m) const {
+        const uint32_t bits = 2u | (pcount >= m ? 1u : 0u) | (m != 0u ? 4u : 0u);
+        return pick(r4 < 8u, m, ((bits >> ((r4 >> 1) & 3u)) & 1u) << 7);
+    }
+
+    // One instruction of every lane whose PC lies in the cartridge window, in
+    // one straight-line pass steered by the one-hot words d2 / d3: loads,
+    // stores, ALU, read-modify-write, transfers, flag instructions,
+    // conditional branches, JMP, PHA / PHP / PLA / PLP.  Only TIA / RIOT
+    // writes, RIOT reads, zero-page pointers in the TIA and decimal mode
+    // branch, each on a 64-bit ballot of the lanes that need it.  Bus
+    // accesses are those of src/cpu.rs in the reference's order; the phantom
+    // write of a read-modify-write is made only where it can be seen (TIA /
+    // RIOT), since on RAM the final write replaces it and on ROM both writes
+    // select the same bank.  A branch's target is computed as an effective
+    // address (PC + offset) whose page-cross fix-up read is an indexed read's;
+    // the taken branch's phantom read of PC has no side effect inside the
+    // window (below the hotspots, outside the RIOT).  A stack access is a
+    // data access at $0100 + SP (RAM or TIA, as zero page).  Lanes of any
+    // other kind (d2 = d3 = 0, br = false) change nothing here; illegal
+    // opcodes return 0 cycles.
+    __device__ __forceinline__ uint32_t compute_lanes(uint32_t b1, uint32_t b2, uint32_t d0, uint32_t d1, uint32_t g,
+                                                      uint32_t h, bool fz) {
         uint32_t cyc = (d0 >> D_CYC_SHIFT) & 7u;
+        const uint32_t mbr = bm(h, H_BR) & mk(fz);
+        const uint32_t mpush = bm(h, H_PUSH), mpull = bm(h, H_PULL), mpreg = bm(h, H_PREG);
         const uint32_t s = (a & bm(g, G_SRC)) | (x & bm(g, G_SRC + 1u)) | (y & bm(g, G_SRC + 2u)) |
-                           (sp & bm(g, G_SRC + 3u));
-        // effective address (every lane computes one; only D_MEM lanes use it)
+                           (sp & bm(g, G_SRC + 3u)) | ((p | FB | FU) & mpreg & mpush);
+        // effective address (every lane computes one; only lanes with a data access use it)
         const uint32_t zpm = bm(d0, 8u);                         // D_ZP: 8-bit address arithmetic
-        uint32_t base = b1 | ((b2 << 8) & ~zpm);
-        if (g & (1u << G_IND)) {
-            // pointer from the zero page; pointer bytes in the TIA read its registers
-            const uint32_t zp = (b1 + (x & bm(g, G_IZX))) & 0xFFu;
-            const uint32_t zq = (zp + 1u) & 0xFFu;
-            uint32_t lo = ld8(ram_off(zp)), hi = ld8(ram_off(zq));
-            if (__ballot(((zp & zq) & 0x80u) == 0u) != 0) {
-                lo = (zp & 0x80u) ? lo : tia_read_lanes(zp);
-                hi = (zq & 0x80u) ? hi : tia_read_lanes(zq);
-            }
-            base = lo | (hi << 8);
+        const bool ind = (g & (1u << G_IND)) != 0u;
+        const uint32_t zp = (b1 + (x & bm(g, G_IZX))) & 0xFFu;
+        const uint32_t zq = (zp + 1u) & 0xFFu;
+        uint32_t lo = ld8(ram_off(zp)), hi = ld8(ram_off(zq));
+        if (__ballot(ind & (((zp & zq) & 0x80u) == 0u)) != 0) {
+            // pointer bytes in the TIA read its registers
+            if (ind & !(zp & 0x80u)) lo = tia_read_lanes(zp);
+            if (ind & !(zq & 0x80u)) hi = tia_read_lanes(zq);
         }
-        const uint32_t ea = (base + ((x & bm(g, G_IX_X)) | (y & bm(g, G_IX_Y)))) & (0xFFFFu ^ (0xFF00u & zpm));
+        const uint32_t mind = mk(ind), mstk = mpush | mpull;
+        uint32_t base = ((lo | (hi << 8)) & mind) | ((b1 | ((b2 << 8) & ~zpm)) & ~mind);
+        base = ((0x100u | ((sp + (1u & mpull)) & 0xFFu)) & mstk) | (base & ~mstk);
+        const uint32_t jbase = base;
+        base = (pc & mbr) | (base & ~mbr);
+        const uint32_t rel = (b1 - ((b1 & 0x80u) << 1)) & mbr;
+        const uint32_t ea = (base + ((x & bm(g, G_IX_X)) | (y & bm(g, G_IX_Y)) | rel)) & (0xFFFFu ^ (0xFF00u & zpm));
+        const bool taken = (mbr != 0u) & (((p & (d1 >> D1_FMASK_SHIFT) & 0xFFu) != 0u) == ((d1 >> D1_FVAL_SHIFT) != 0u));
         const bool cross = ((base ^ ea) & 0xFF00u) != 0u;
-        const bool cross_fix = ((d0 & D_DUMMY_CROSS) != 0u) & cross;
+        const bool cross_fix = (((d0 & D_DUMMY_CROSS) != 0u) | taken) & cross;
         touch_if(((d0 & D_DUMMY_ALWAYS) != 0u) | cross_fix, (base & 0xFF00u) | (ea & 0xFFu));
-        cyc += cross_fix ? 1u : 0u;
-        // operand: immediate, register or the data read
+        cyc += (cross_fix ? 1u : 0u) + (taken ? 1u : 0u);
+        pc = pick(taken, ea, pick((h & (1u << H_JMP)) != 0u, jbase, pc));
+        sp = (sp + (1u & mpull) - (1u & mpush)) & 0xFFu;
+        // operand: immediate, register or the data read.  The read is one
+        // load from the ROM window, a RAM mirror, the TIA register's image
+        // byte or a zero byte (the flat array's padding); the RIOT branches.
         uint32_t v = (d0 & D_IMM) ? b1 : s;
         {
-            const bool rd = (d0 & D_RD) != 0u;
+            const bool rd = ((d0 & D_RD) != 0u) | (mpull != 0u);
             const uint32_t a13 = ea & 0x1FFFu;
             const bool rom = (a13 & 0x1000u) != 0u;
             hotspot_if(rd, a13);
             const bool ram = (a13 & 0x1E80u) == 0x0080u;
-            // TIA registers are read without side effects: their byte is loaded
-            // beside the memory byte and selected; only the RIOT branches
             const bool tia = (a13 & 0x1080u) == 0u;
-            const uint32_t m = ld8(rd ? (rom ? ROM0 + (bank << 12) + (a13 & 0xFFFu) : ram_off(a13)) : junk);
-            const uint32_t t = ld8(rd & tia ? tia_read_off(a13) : junk);
-            v = rd ? ((rom | ram) ? m : (tia ? tia_read_val(a13, t) : 0u)) : v;
+            const uint32_t r4 = a13 & 0x0Fu;
+            uint32_t ad = pick(tia, img + S_CX + tia_lut(r4), ZERO_BYTE);
+            ad = pick(ram, ram_off(a13), ad);
+            ad = pick(rom, ROM0 + (bank << 12) + (a13 & 0xFFFu), ad);
+            uint32_t m = ld8(ad);
+            m = pick(tia, tia_fix(r4, m), m);
+            v = pick(rd, m, v);
             const bool riot = rd & ((a13 & 0x1280u) == 0x0280u);
             if (__ballot(riot) != 0) {
                 if (riot) v = pia_read(a13);
@@ -582,13 +621,14 @@ struct Emu {
         const uint32_t inc = (v + (g >> G_DELTA)) & 0xFFu;       // bits above the delta fall off with the mask
         const uint32_t r = (lg & bm(g, G_R_LOG)) | (ar & bm(g, G_R_ARI)) | (shl & bm(g, G_R_SHL)) |
                            (shr & bm(g, G_R_SHR)) | (inc & bm(g, G_R_INC)) | (s & bm(g, G_R_SRC));
-        // status: N, V, Z, C candidates, then the descriptor's clear / take / set masks
+        // status: N, V, Z, C candidates, then the descriptor's clear / take / set
+        // masks; PLP takes every flag from the pulled byte
         const uint32_t mbit = bm(g, G_BIT);
         const uint32_t nv = (v & mbit) | (r & ~mbit);
         const uint32_t vf = (((~(s ^ vv) & (s ^ ar)) >> 1) & FV & ~mbit) | (v & FV & mbit);
         const uint32_t cf = (((sum >> 8) & bm(g, G_C_SUM)) | ((v >> 7) & bm(g, G_C_B7)) | (v & bm(g, G_C_B0))) & FC;
         const uint32_t cand = (nv & FN) | vf | (r == 0u ? FZ : 0u) | cf;
-        p = (p & ~(h & 0xFFu)) | (cand & (h >> H_TAKE)) | ((h >> H_SET) & 0xFFu);
+        p = (p & ~(h & 0xFFu)) | (cand & (h >> H_TAKE)) | ((h >> H_SET) & 0xFFu) | (v & ~(FB | FU) & mpreg & mpull);
         const uint32_t ma = bm(g, G_DST), mx = bm(g, G_DST + 1u), my = bm(g, G_DST + 2u), ms = bm(g, G_DST + 3u);
         a = (r & ma) | (a & ~ma);
         x = (r & mx) | (x & ~mx);
@@ -598,7 +638,7 @@ struct Emu {
         if (__ballot(dec) != 0) decimal_lanes(dec, (d0 & D_INV) != 0u, a0, v, p0);
         // write-back
         {
-            const bool wr = (d0 & D_WR) != 0u;
+            const bool wr = ((d0 & D_WR) != 0u) | (mpush != 0u);
             const uint32_t a13 = ea & 0x1FFFu;
             hotspot_if(wr, a13);
             const bool rom = (a13 & 0x1000u) != 0u;
@@ -624,11 +664,23 @@ struct Emu {
 
     // Executes one decoded instruction (pc already points past it).  Returns
     // the cycle count, or 0 for an illegal opcode (src/cpu.rs:1139-1141).
-    __device__ __forceinline__ uint32_t execute(uint32_t b1, uint32_t b2, const uint4 &d) {
+    __device__ __forceinline__ uint32_t execute(uint32_t b1, uint32_t b2, const uint4 &d, bool fz) {
+        const uint32_t cls = d.x & 3u;
+        // JSR, RTS, RTI, BRK, JMP (ind), and branches outside the window
+        const bool slow = ((cls == CLS_SPECIAL) & ((d.w >> H_JMP) == 0u)) | ((cls == CLS_BRANCH) & !fz);
+        uint32_t cyc = compute_lanes(b1, b2, d.x, d.y, d.z, d.w, fz);
+        if (__ballot(slow) != 0) {
+            if (slow) cyc = execute_slow(b1, b2, d);
+        }
+        return cyc;
+    }
+
+    // The instructions compute_lanes leaves: every bus access in the
+    // reference's order (src/cpu.rs:949-1137).
+    __device__ __forceinline__ uint32_t execute_slow(uint32_t b1, uint32_t b2, const uint4 &d) {
         const uint32_t d0 = d.x, d1 = d.y;
         uint32_t cyc = (d0 >> D_CYC_SHIFT) & 7u;
         const uint32_t cls = d0 & 3u;
-        if (cls == CLS_COMPUTE) return compute_lanes(b1, b2, d0, d.z, d.w);
         if (cls == CLS_BRANCH) {
             const uint32_t fm = (d1 >> D1_FMASK_SHIFT) & 0xFFu;
             const bool taken = ((p & fm) != 0u) == ((d1 >> D1_FVAL_SHIFT) != 0u);
@@ -640,10 +692,10 @@ struct Emu {
             pc = taken ? np : pc;
             return cyc;
         }
-        if (cls == CLS_ILLEGAL) return 0u;
-        // stack / jumps (src/cpu.rs:949-1137); phantom stack reads have no side effect
+        // JSR, RTS, RTI, BRK, JMP (ind) (src/cpu.rs:949-1137); JMP absolute
+        // and the pushes / pulls run in compute_lanes; phantom stack reads
+        // have no side effect
         switch ((d1 >> D1_SPEC_SHIFT) & 15u) {
-        case SP_JMP: pc = b1 | (b2 << 8); break;
         case SP_JMPI: {
             const uint32_t ptr = b1 | (b2 << 8);
             const uint32_t lo = bus_read(ptr);
@@ -697,10 +749,7 @@ struct Emu {
             pc = lo | (hi << 8);
             break;
         }
-        case SP_PHA: push(a); break;
-        case SP_PHP: push(p | FB | FU); break;
-        case SP_PLA: a = pull(); set_zn(a); break;
-        default: p = pull() & ~(FB | FU); break;            // SP_PLP
+        default: break;
         }
         return cyc;
     }
@@ -980,7 +1029,7 @@ struct Emu {
             uint4 gd = ld128(DESC0 + op * 16u);
             if (!fz) fetch(gb1, gb2, gd, gop);
             pc = (pc0 + ((gd.x >> D_LEN_SHIFT) & 3u)) & 0xFFFFu;
-            cyc = execute(gb1, gb2, gd);
+            cyc = execute(gb1, gb2, gd, fz);
         }
         return cyc;
     }

@@ -185,6 +185,9 @@ inline void build_tables(uint32_t *out) {
         const uint32_t d0 = (uint32_t)d, d1 = (uint32_t)(d >> 32);
         uint32_t g = 0, h = 0;
         describe_lanes(d0, d1, g, h);
+        // stack / jump instructions run micro-programs here: the frame
+        // kernel's straight-line stack and jump bits do not apply
+        if ((d0 & 3u) == CLS_SPECIAL) g = h = 0;
         if ((d0 & 3u) == CLS_BRANCH) h = d1 & 0xFFFF0000u;
         ops[4 * c] = rdesc_of(c);
         ops[4 * c + 1] = d0;

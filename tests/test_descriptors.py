@@ -1,10 +1,12 @@
 """The per-opcode descriptor table that steers the kernels (fa_descriptor_table,
 csrc/fa_decode.hpp), checked on the CPU against the documented instruction set
 of tests/opcode_table.py: instruction length and base cycles per addressing
-mode, and -- for every compute-class opcode -- the one-hot words d2 / d3
-select the result, the destination register and the status bits that the
-instruction's textbook definition gives, on random registers and operands
-(binary mode; decimal ADC / SBC is checked end-to-end by the GPU parity tests).
+mode, and -- for every compute-class opcode and for PHA / PHP / PLA / PLP --
+the one-hot words d2 / d3 select the source, the result, the destination
+register, the stack-pointer step and the status bits that the instruction's
+textbook definition gives, on random registers and operands (binary mode;
+decimal ADC / SBC is checked end-to-end by the GPU parity tests); branches and
+JMP carry their own d3 bit, the other stack / jump instructions none.
 """
 import ctypes
 
@@ -24,6 +26,8 @@ G_SRC, G_DST, G_R_LOG, G_R_ARI, G_R_SHL, G_R_SHR, G_R_INC, G_R_SRC = 0, 4, 8, 9,
 G_CIN_SHL, G_CIN_SHR, G_DELTA, G_IX_X, G_IX_Y, G_IZX, G_IND = 14, 15, 16, 24, 25, 26, 27
 G_BIT, G_C_SUM, G_C_B7, G_C_B0 = 28, 29, 30, 31
 H_TAKE, H_SET, H_M_AND, H_M_XOR, H_M_LD = 8, 16, 24, 25, 26
+H_BR, H_JMP, H_PUSH, H_PULL, H_PREG = 27, 28, 29, 30, 31
+PUSH_PULL = {"PHA", "PHP", "PLA", "PLP"}
 FC, FZ, FI, FD, FV, FN = 0x01, 0x02, 0x04, 0x08, 0x40, 0x80
 
 
@@ -46,8 +50,12 @@ def lanes_select(d, a, x, y, sp, p, v_in):
     words: returns (a, x, y, sp, p, r)."""
     d0, _, g, h = (int(w) for w in d)
     m = 0xFFFFFFFF
-    s = (a & _bit(g, G_SRC)) | (x & _bit(g, G_SRC + 1)) | (y & _bit(g, G_SRC + 2)) | (sp & _bit(g, G_SRC + 3))
-    v = v_in if (d0 & (1 << 14)) else (v_in if (d0 & D_IMM) else s)     # D_RD: memory operand
+    push, pull, preg = _bit(h, H_PUSH), _bit(h, H_PULL), _bit(h, H_PREG)
+    s = ((a & _bit(g, G_SRC)) | (x & _bit(g, G_SRC + 1)) | (y & _bit(g, G_SRC + 2)) | (sp & _bit(g, G_SRC + 3))
+         | ((p | 0x30) & preg & push))
+    sp = (sp + (1 & pull) - (1 & push)) & 0xFF
+    # D_RD or a pull: memory operand
+    v = v_in if ((d0 & (1 << 14)) or pull) else (v_in if (d0 & D_IMM) else s)
     lg = (a & v & _bit(h, H_M_AND)) | ((a ^ v) & _bit(h, H_M_XOR)) | (v & _bit(h, H_M_LD))
     vv = v ^ (0xFF & _bit(d0, D_INV_BIT))
     cin = p & FC
@@ -63,7 +71,7 @@ def lanes_select(d, a, x, y, sp, p, v_in):
     vf = ((((~(s ^ vv)) & (s ^ ar)) >> 1) & FV & ~mb) | (v & FV & mb)
     cf = (((sm >> 8) & _bit(g, G_C_SUM)) | ((v >> 7) & _bit(g, G_C_B7)) | (v & _bit(g, G_C_B0))) & FC
     cand = (nv & FN) | vf | (FZ if r == 0 else 0) | cf
-    p2 = ((p & ~(h & 0xFF)) | (cand & (h >> H_TAKE)) | ((h >> H_SET) & 0xFF)) & 0xFF
+    p2 = ((p & ~(h & 0xFF)) | (cand & (h >> H_TAKE)) | ((h >> H_SET) & 0xFF) | (v & 0xCF & preg & pull)) & 0xFF
     regs = []
     for k, old in enumerate((a, x, y, sp)):
         mk = _bit(g, G_DST + k)
@@ -136,6 +144,16 @@ def textbook(mn, mode, a, x, y, sp, p, v):
     elif mn in ("CLC", "SEC", "CLI", "SEI", "CLV", "CLD", "SED"):
         bit = {"C": FC, "I": FI, "V": FV, "D": FD}[mn[2]]
         p = (p | bit) if mn[0] == "S" else (p & ~bit)
+    elif mn in ("PHA", "PHP"):
+        out = a if mn == "PHA" else (p | 0x30)
+        sp = (sp - 1) & 0xFF
+    elif mn in ("PLA", "PLP"):
+        sp = (sp + 1) & 0xFF
+        if mn == "PLA":
+            a = v
+            p = zn(p, v)
+        else:
+            p = v & 0xCF            # B and U are not stored in P
     else:
         assert mn == "NOP", mn
     return a, x, y, sp, p, out
@@ -167,15 +185,28 @@ def test_addressing_words_match_the_modes(table):
         assert got == want, f"{mn} {mode}"
 
 
+def test_branch_and_jump_words(table):
+    for op, (mn, mode, _) in OPCODES.items():
+        g, h = int(table[op, 2]), int(table[op, 3])
+        if mn in BRANCH:
+            assert (g, h) == (0, 1 << H_BR), mn
+        elif mn == "JMP" and mode == "abs":
+            assert (g, h) == (0, 1 << H_JMP), mn
+        elif mn in STACK_JUMP and mn not in PUSH_PULL:
+            assert (g, h) == (0, 0), f"{mn} {mode}"
+        else:
+            assert h >> H_BR & 3 == 0, f"{mn} {mode}"
+
+
 def test_one_hot_words_give_the_textbook_effect(table):
     rng = np.random.default_rng(7)
     for op, (mn, mode, _) in sorted(OPCODES.items()):
-        if mn in BRANCH or mn in STACK_JUMP:
+        if mn in BRANCH or (mn in STACK_JUMP and mn not in PUSH_PULL):
             continue
         for _ in range(400):
             a, x, y, sp, v = (int(q) for q in rng.integers(0, 256, 5))
             p = int(rng.integers(0, 256)) & ~FD            # binary mode
-            if mode == "imm" or mode in ("zp", "zpx", "zpy", "abs", "abx", "aby", "izx", "izy"):
+            if mode in ("imm", "zp", "zpx", "zpy", "abs", "abx", "aby", "izx", "izy") or mn in ("PLA", "PLP"):
                 operand = v
             else:
                 operand = None
