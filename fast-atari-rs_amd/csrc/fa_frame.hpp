@@ -51,28 +51,31 @@ __device__ __forceinline__ uint32_t run_frame(Emu &e, uint32_t &st, uint32_t &cp
         } else {
             --hold;
         }
-        if (in_a && !(e.vsync & 2u)) in_a = false;
-        if (!in_a && (e.vsync & 2u)) break;
+        // loop A while VSYNC is held, loop B until it rises
+        const bool v2 = (e.vsync & 2u) != 0u;
+        in_a = in_a & v2;
+        if (!in_a & v2) break;
+        bool stop = false;
         if (e.wsync) {
             uint32_t s = e.tia_skip();
             if (s == 0u) { st = ENV_VSYNC_HELD; break; }   // only a clock >= 228 could stall here
             e.pia_tick(s);
             cycles += s;
         } else {
+            // an illegal opcode (0 cycles) changes nothing below and ends the frame
             uint32_t c = e.exec<SORTED>(op, b1, b2);
-            if (c == 0u) { st = ENV_ILLEGAL; break; }
             e.fetch_ahead(op, b1, b2);
             cycles += c;
             cpu_cyc += c;
-            instr += 1u;
+            stop = c == 0u;
+            instr += stop ? 0u : 1u;
+            st = stop ? ENV_ILLEGAL : st;
             e.tia_tick(c);
             e.pia_tick(c);
         }
-        if (in_a) {
-            if (cycles > LOOPA_CAP) { st = ENV_VSYNC_HELD; break; }
-        } else if (cycles > LOOPB_CAP) {
-            break;
-        }
+        const bool capa = in_a & (cycles > LOOPA_CAP);
+        st = capa ? ENV_VSYNC_HELD : st;
+        if (stop | capa | (!in_a & (cycles > LOOPB_CAP))) break;
     }
     return cycles;
 }
