@@ -25,10 +25,15 @@
  * An env with a non-zero error is frozen: later frames do nothing.
  *
  * Parity unpinned (stated only by the reference's text, restated here and
- * reviewed, no independent check): the HeadlessBus flat-array read quirk --
- * RAM-mirror reads outside $80-$FF / $180-$1FF return 0 (src/atari.rs:386-395,
- * 503-507) -- and the frame-loop caps (src/atari.rs:594).  Everything else is
- * pinned by tests/test_oracle_*.py and tests/golden/.
+ * reviewed, no independent check): the headless TIA's paddle model -- the
+ * counter's ramp and the 200 / 50 / 128 paddle values of set_action
+ * (src/headless_tia.rs, src/atari.rs:571-582) -- and, on the rendering path
+ * (orc_render.inc), the player RESPx formula, the RESMx / RESBL offset and
+ * the visible-area resizer constants (src/tia.rs:76-80, 536-555).  Everything
+ * else is pinned by tests/test_oracle_*.py (including the flat-array padding
+ * reads of src/atari.rs:386-395, 503-507 and the frame-loop caps of
+ * src/atari.rs:588-596, by closed forms and the 2600 address map) and
+ * tests/golden/.
  *
  * Licence note: the reference is GPL-3.0; this restatement follows its
  * algorithm and is kept under the same licence terms.

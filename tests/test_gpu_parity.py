@@ -90,9 +90,11 @@ def test_config2_per_frame(fa, orc, W):
 @pytest.mark.parametrize("sorted_", [False, True])
 def test_config3_branchy(fa, orc, W, sorted_):
     """Config 3 (F6, no WSYNC, all 151 opcodes, data-dependent branches),
-    sorted and unsorted (tests/test_cuda_sorted.rs shape)."""
+    sorted and unsorted (tests/test_cuda_sorted.rs shape), every frame past
+    the first 30 after reset, over which the waves' divergence grows to its
+    steady level."""
     rom = W.rom_for(3)
-    n, frames = 1024, 12
+    n, frames = 1024, 48
     gpu, ref = _pair(fa, orc, rom, n, sorted_)
     acts = W.ActionStream(W.config_seed(3), n)
     for f in range(frames):
@@ -102,12 +104,13 @@ def test_config3_branchy(fa, orc, W, sorted_):
         assert_same(gpu, ref, f"frame {f}")
 
 
-@pytest.mark.parametrize("cfg,frames,k", [(2, 1000, 16), (3, 300, 10)])
-def test_full_size_sampled(fa, orc, W, cfg, frames, k):
+@pytest.mark.parametrize("cfg,frames,k,checks", [(2, 1000, 16, 5), (3, 300, 10, 6)])
+def test_full_size_sampled(fa, orc, W, cfg, frames, k, checks):
     """Configs 2 and 3 at their full size (65,536 envs; 1,000 F8 frames as
     step_multi K = 16, 300 branchy frames), every env on the GPU and a seeded
     sample of 1,024 envs replayed on the oracle with the same action columns:
-    state, counters and status equal half-way and at the end."""
+    state, counters, status and obs equal at `checks` launch boundaries spread
+    over the run (the last one at the end)."""
     rom = W.rom_for(cfg)
     n = 65536
     gpu = fa.BatchAtariGpu(rom, n)
@@ -118,8 +121,8 @@ def test_full_size_sampled(fa, orc, W, cfg, frames, k):
     ref.reset()
     d_act, d_obs = fa.DeviceBuffer(acts.nbytes), fa.DeviceBuffer(n * 128)
     d_act.upload(acts)
-    half = (frames // 2 // k) * k
-    for lo, hi in ((0, half), (half, frames)):
+    bounds = [0] + [min(frames, (frames * (i + 1) // checks + k - 1) // k * k) for i in range(checks)]
+    for lo, hi in zip(bounds[:-1], bounds[1:]):
         for f in range(lo, hi, k):
             gpu.step_device(d_act.at(f * n), min(k, hi - f), d_obs.ptr)
         gpu.sync()
