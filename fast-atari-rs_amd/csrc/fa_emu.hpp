@@ -132,7 +132,7 @@ __device__ __forceinline__ void timer_tick_slow(T &t, uint32_t n) {
 template <class T>
 __device__ __forceinline__ void timer_tick(T &t, uint32_t n) {
     const bool post = t.tval == 0 && t.tuf;
-    if (!post && t.tps > n) { t.tps -= n; return; }       // common case: inside a prescaler period
+    if (__builtin_expect(!post && t.tps > n, 1)) { t.tps -= n; return; }   // common case: inside a prescaler period
     timer_tick_slow(t, n);
 }
 // Timer write (Pia::write with addr & 0x10, src/pia.rs:202-212).
@@ -373,14 +373,14 @@ struct Emu {
         wsync = pick(r == 0x02u, 1u, wsync);
         // RESP0..RESBL: position from the clock at the start of the instruction
         const bool resp = (r - 0x10u) < 5u;
-        if (__ballot(resp) != 0) {
+        if (__builtin_expect(__ballot(resp) != 0, 0)) {
             const uint32_t pos = resp_pos_of(clock);
             const uint32_t po = img + S_POS + 2u * (r - 0x10u);
             st8(resp ? po : junk, pos);
             st8(resp ? po + 1u : junk, pos >> 8);
         }
         const bool gp0 = r == 0x1Bu, gp1 = r == 0x1Cu;
-        if (__ballot(gp0 | gp1) != 0) {
+        if (__builtin_expect(__ballot(gp0 | gp1) != 0, 0)) {
             // GRP0: grp0_old = grp0; GRP1: grp1_old = grp1 and enabl_old = enabl
             const uint32_t g = ld8(img + (gp0 ? S_GRP0 : S_GRP1));
             st8((gp0 | gp1) ? img + (gp0 ? S_GRP0_OLD : S_GRP1_OLD) : junk, g);
@@ -388,7 +388,7 @@ struct Emu {
             const uint32_t en = ld8(img + S_ENABL);
             st8(gp1 ? img + S_ENABL_OLD : junk, en);
         }
-        if (__ballot((r - 0x2Au) < 3u) != 0) {
+        if (__builtin_expect(__ballot((r - 0x2Au) < 3u) != 0, 0)) {
             if (r == 0x2Au) {
                 // HMOVE: pos - ((hm as i8) >> 4), rem_euclid 160
                 for (uint32_t i = 0; i < 5u; ++i) {
@@ -549,11 +549,12 @@ struct Emu {
     // RIOT), since on RAM the final write replaces it and on ROM both writes
     // select the same bank.  A branch's target is computed as an effective
     // address (PC + offset) whose page-cross fix-up read is an indexed read's;
-    // the taken branch's phantom read of PC has no side effect inside the
-    // window (below the hotspots, outside the RIOT).  A stack access is a
-    // data access at $0100 + SP (RAM or TIA, as zero page).  Lanes of any
-    // other kind (d2 = d3 = 0, br = false) change nothing here; illegal
-    // opcodes return 0 cycles.
+    // the taken branch's phantom read of PC has no side effect for a branch
+    // that starts below $xFF0 of the window (below the hotspots, outside the
+    // RIOT).  A stack access is a data access at $0100 + SP (RAM or TIA, as
+    // zero page).  Lanes of any other kind (d2 = d3 = 0; branches outside
+    // the window, whose H_BR is masked off by fz) change nothing here;
+    // illegal opcodes return 0 cycles.
     __device__ __forceinline__ uint32_t compute_lanes(uint32_t b1, uint32_t b2, uint32_t d0, uint32_t d1, uint32_t g,
                                                       uint32_t h, bool fz) {
         uint32_t cyc = (d0 >> D_CYC_SHIFT) & 7u;
@@ -567,7 +568,7 @@ struct Emu {
         const uint32_t zp = (b1 + (x & bm(g, G_IZX))) & 0xFFu;
         const uint32_t zq = (zp + 1u) & 0xFFu;
         uint32_t lo = ld8(ram_off(zp)), hi = ld8(ram_off(zq));
-        if (__ballot(ind & (((zp & zq) & 0x80u) == 0u)) != 0) {
+        if (__builtin_expect(__ballot(ind & (((zp & zq) & 0x80u) == 0u)) != 0, 0)) {
             // pointer bytes in the TIA read its registers
             if (ind & !(zp & 0x80u)) lo = tia_read_lanes(zp);
             if (ind & !(zq & 0x80u)) hi = tia_read_lanes(zq);
@@ -605,7 +606,7 @@ struct Emu {
             m = pick(tia, tia_fix(r4, m), m);
             v = pick(rd, m, v);
             const bool riot = rd & ((a13 & 0x1280u) == 0x0280u);
-            if (__ballot(riot) != 0) {
+            if (__builtin_expect(__ballot(riot) != 0, 0)) {
                 if (riot) v = pia_read(a13);
             }
         }
@@ -635,7 +636,7 @@ struct Emu {
         y = (r & my) | (y & ~my);
         sp = (r & ms) | (sp & ~ms);
         const bool dec = ((d0 & D_DEC) != 0u) & ((p0 & FD) != 0u);
-        if (__ballot(dec) != 0) decimal_lanes(dec, (d0 & D_INV) != 0u, a0, v, p0);
+        if (__builtin_expect(__ballot(dec) != 0, 0)) decimal_lanes(dec, (d0 & D_INV) != 0u, a0, v, p0);
         // write-back
         {
             const bool wr = ((d0 & D_WR) != 0u) | (mpush != 0u);
@@ -645,7 +646,7 @@ struct Emu {
             const bool ram = (a13 & 0x1280u) == 0x0080u;
             st8(wr & ram ? ram_off(a13) : junk, r);
             const bool io = wr & !rom & !ram;
-            if (__ballot(io) != 0) {
+            if (__builtin_expect(__ballot(io) != 0, 0)) {
                 // The read-modify-write's first write (the old value) leaves a
                 // trace only on VSYNC (edge), VBLANK (paddle dump), GRP0 / GRP1
                 // (the old-copy shift) and HMOVE (motion); on every other TIA
@@ -653,7 +654,7 @@ struct Emu {
                 const uint32_t tr = a13 & 0x3Fu;
                 const bool lasting = !(a13 & 0x200u) & ((tr <= 0x01u) | (tr == 0x1Bu) | (tr == 0x1Cu) | (tr == 0x2Au));
                 const bool phantom = io & ((d0 & D_RMWMEM) != 0u) & lasting;
-                if (__ballot(phantom) != 0) {
+                if (__builtin_expect(__ballot(phantom) != 0, 0)) {
                     if (phantom) io_write(a13, v);
                 }
                 if (io) io_write(a13, r);
@@ -669,7 +670,7 @@ struct Emu {
         // JSR, RTS, RTI, BRK, JMP (ind), and branches outside the window
         const bool slow = ((cls == CLS_SPECIAL) & ((d.w >> H_JMP) == 0u)) | ((cls == CLS_BRANCH) & !fz);
         uint32_t cyc = compute_lanes(b1, b2, d.x, d.y, d.z, d.w, fz);
-        if (__ballot(slow) != 0) {
+        if (__builtin_expect(__ballot(slow) != 0, 0)) {
             if (slow) cyc = execute_slow(b1, b2, d);
         }
         return cyc;
@@ -715,7 +716,7 @@ struct Emu {
             push(ret & 0xFFu);
             const uint32_t a13 = ret & 0x1FFFu;
             uint32_t hi = b2;
-            if (__ballot((a13 & 0x1000u) == 0u) != 0) {           // some lane's JSR lies outside the ROM
+            if (__builtin_expect(__ballot((a13 & 0x1000u) == 0u) != 0, 0)) {   // some lane's JSR lies outside the ROM
                 const bool ram = (a13 & 0x1E80u) == 0x0080u, tia = (a13 & 0x1080u) == 0u;
                 const uint32_t m = ld8(ram ? ram_off(a13) : tia_read_off(a13));
                 hi = ram ? m : (tia ? tia_read_val(a13, m) : b2);

@@ -56,7 +56,7 @@ __device__ __forceinline__ uint32_t run_frame(Emu &e, uint32_t &st, uint32_t &cp
         in_a = in_a & v2;
         if (!in_a & v2) break;
         bool stop = false;
-        if (e.wsync) {
+        if (__builtin_expect(e.wsync != 0u, 0)) {
             uint32_t s = e.tia_skip();
             if (s == 0u) { st = ENV_VSYNC_HELD; break; }   // only a clock >= 228 could stall here
             e.pia_tick(s);
