@@ -28,3 +28,71 @@ def jsr_rom():
     put(0xF0C0, [0xE6, 0x83, 0x4C, 0x00, 0xF1])                   # INC $83; JMP $F100
     rom[0xFFC], rom[0xFFD] = 0x00, 0xF0
     return bytes(rom)
+
+
+def stack_rom():
+    """Fixed 4 KB ROM for the stack instructions at every stack pointer: each
+    frame takes SP from a per-env walk (SWCHA EOR $90 + $1D, written back to
+    $90) and runs PHA, PHP, PHA, PLA, PLP, PLA there, so over the frames the
+    pushes and pulls sweep the stack page, including $0100-$017F, where they
+    write and read TIA registers (VSYNC, WSYNC, RESPx, HMOVE, ... on a push;
+    collision and input latches on a pull).  Then, with SP moved into the RAM
+    half ($90 | $88), it runs JSR / RTS (PHP / PLP and TSX inside the
+    subroutine) and BRK / RTI (the handler pushes and pulls A), whose pushes
+    land on the zero-page RAM they mirror, and restores SP = $FF."""
+    rom = bytearray([0xEA] * 4096)
+
+    def put(addr, prog):
+        rom[addr - 0xF000: addr - 0xF000 + len(prog)] = bytes(prog)
+
+    put(0xF000, [0x78, 0xD8, 0xA2, 0xFF, 0x9A,                     # SEI; CLD; LDX #$FF; TXS
+                 0xA9, 0x00, 0x85, 0x90,                           # $90 = 0
+                 0x4C, 0x00, 0xF1])                                # JMP $F100
+    put(0xF100, [0xA9, 0x02, 0x85, 0x00, 0xA9, 0x00, 0x85, 0x00,   # VSYNC on / off: the frame boundary
+                 0xAD, 0x80, 0x02, 0x45, 0x90, 0x69, 0x1D, 0x85, 0x90,   # LDA SWCHA; EOR $90; ADC #$1D; STA $90
+                 0xAA, 0x9A,                                       # TAX; TXS
+                 0xA5, 0x91, 0x48,                                 # LDA $91; PHA
+                 0x08,                                             # PHP
+                 0xA5, 0x90, 0x48,                                 # LDA $90; PHA
+                 0x68, 0x85, 0x93,                                 # PLA; STA $93
+                 0x28,                                             # PLP
+                 0x68, 0x85, 0x94,                                 # PLA; STA $94
+                 0xA5, 0x90, 0x09, 0x88, 0xAA, 0x9A,               # LDA $90; ORA #$88; TAX; TXS
+                 0x20, 0x00, 0xF2,                                 # JSR $F200
+                 0x00, 0xEA,                                       # BRK (+ padding byte)
+                 0xE6, 0x91,                                       # INC $91
+                 0xA2, 0xFF, 0x9A,                                 # LDX #$FF; TXS
+                 0x4C, 0x00, 0xF1])                                # JMP $F100
+    put(0xF200, [0xBA, 0x86, 0x95, 0x08, 0x28, 0x60])             # TSX; STX $95; PHP; PLP; RTS
+    put(0xF300, [0x48, 0x8A, 0x85, 0x96, 0x68, 0x40])             # PHA; TXA; STA $96; PLA; RTI
+    rom[0xFFC], rom[0xFFD] = 0x00, 0xF0
+    rom[0xFFE], rom[0xFFF] = 0x00, 0xF3
+    return bytes(rom)
+
+
+def branch_hotspot_rom():
+    """F8 ROM whose taken branches reach the bank-switch hotspots through
+    their dummy reads (src/cpu.rs: a taken branch reads PC, a page-crossing
+    one then reads the target's low byte in PC's page).  Bank 1: BNE at $FF6E
+    (inside the window) to $FEF8 crosses from page $FF to $FE, so its fix-up
+    read is $FFF8 and the target comes from bank 0 (marker $80; bank 1's
+    $FEF8 would bump $81).  Bank 0: BNE at $FFF7 (past the window) reads PC =
+    $FFF9 when taken, so its target $FF80 comes from bank 1 (marker $82;
+    bank 0's $FF80 would bump $83).  Each frame runs both, then VSYNC."""
+    rom = bytearray([0xEA] * 8192)
+    for b in range(2):
+        o = b * 4096
+
+        def put(addr, prog):
+            rom[o + addr - 0xF000: o + addr - 0xF000 + len(prog)] = bytes(prog)
+
+        put(0xF000, [0xA2, 0xFF, 0x9A, 0x4C, 0x00, 0xF1])                  # LDX #$FF; TXS; JMP $F100
+        put(0xF100, [0xA9, 0x02, 0x85, 0x00, 0xA9, 0x00, 0x85, 0x00,       # VSYNC on / off
+                     0xAD, 0xF9, 0xFF,                                     # LDA $FFF9: select bank 1
+                     0x4C, 0x6C, 0xFF])                                    # JMP $FF6C
+        put(0xFF6C, [0xA2, 0x01, 0xD0, 0x88])                             # LDX #1; BNE -$78 (-> $FEF8)
+        put(0xFEF8, [0xE6, 0x80 + b, 0x4C, 0xF5, 0xFF])                   # INC $80 / $81; JMP $FFF5
+        put(0xFFF5, [0xA2, 0x01, 0xD0, 0x87])                             # LDX #1; BNE -$79 (-> $FF80)
+        put(0xFF80, [0xE6, 0x83 - b, 0x4C, 0x00, 0xF1])                   # INC $83 / $82; JMP $F100
+        rom[o + 0xFFC: o + 0x1000] = bytes([0x00, 0xF0, 0x00, 0xF0])
+    return bytes(rom)

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from gpu_util import assert_same
-from romkit import jsr_rom
+from romkit import branch_hotspot_rom, jsr_rom, stack_rom
 
 pytestmark = pytest.mark.gpu
 
@@ -359,3 +359,39 @@ def test_bad_lengths_raise(fa, W):
     with pytest.raises(fa.FastAtariError):
         gpu.upload_states(st)
     assert gpu.num_envs() == 8
+
+
+@pytest.mark.parametrize("sorted_", [False, True])
+def test_stack_instructions_at_every_stack_pointer(fa, orc, W, sorted_):
+    """PHA / PHP / PLA / PLP (straight-line stage) and JSR / RTS / BRK / RTI
+    (slow pass) with the stack pointer swept over the whole stack page by a
+    per-env walk, so that pushes and pulls reach the TIA registers at
+    $0100-$017F: every frame equal to the oracle."""
+    rom = stack_rom()
+    n, frames = 256, 96
+    gpu, ref = _pair(fa, orc, rom, n, sorted_)
+    acts = W.ActionStream(0x57AC, n)
+    sps = set()
+    for f in range(frames):
+        a = acts.next()
+        gpu.step(a)
+        ref.run(a[None, :])
+        assert_same(gpu, ref, f"frame {f}")
+        sps.update(int(v) for v in ref.states()[:, 8 + 0x10])   # RAM $90: this frame's stack pointer
+    assert len([v for v in sps if v < 0x80]) > 64, "the walk should reach the TIA half of the stack page"
+
+
+@pytest.mark.parametrize("sorted_", [False, True])
+def test_branch_dummy_reads_switch_banks(fa, orc, sorted_):
+    """A taken branch's dummy reads select F8 banks: the page-cross fix-up
+    read of a branch inside the window (straight-line stage) and the PC read
+    of a branch past it (slow pass); the targets come from the switched-to
+    bank (markers $80 and $82 count, $81 and $83 stay 0)."""
+    gpu, ref = _pair(fa, orc, branch_hotspot_rom(), 64, sorted_)
+    for f in range(4):
+        a = np.zeros(64, np.uint8)
+        gpu.step(a)
+        ref.run(a[None, :])
+        assert_same(gpu, ref, f"frame {f}")
+    ram = ref.states()[0, 8:12]
+    assert ram[0] > 0 and ram[2] > 0 and ram[1] == 0 and ram[3] == 0, ram
