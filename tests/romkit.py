@@ -96,3 +96,37 @@ def branch_hotspot_rom():
         put(0xFF80, [0xE6, 0x83 - b, 0x4C, 0x00, 0xF1])                   # INC $83 / $82; JMP $F100
         rom[o + 0xFFC: o + 0x1000] = bytes([0x00, 0xF0, 0x00, 0xF0])
     return bytes(rom)
+
+
+def io_rom():
+    """Fixed 4 KB ROM whose TIA / RIOT accesses use a per-env index: X = a
+    per-env walk ($90 + SWCHA's joystick nibble + 7 each frame) & 15 selects the TIA read register ($00,X and its $30
+    mirror, and $08,X: paddles, INPT4/5 and the constant registers), the
+    RIOT register ($0280,X: ports, DDRs, INTIM, INSTAT and their mirrors;
+    $0284,X every line), the timer interval written ($0294,X: TIM1T ..
+    T1024T for X < 12, the port / DDR registers above), and VBLANK gets a
+    per-env byte (bit 7 dumps the paddles).  A frame is VSYNC, those
+    accesses, then 32 WSYNC lines reading INTIM / INSTAT and the paddles."""
+    rom = bytearray([0xEA] * 4096)
+
+    def put(addr, prog):
+        rom[addr - 0xF000: addr - 0xF000 + len(prog)] = bytes(prog)
+
+    put(0xF000, [0x78, 0xD8, 0xA2, 0xFF, 0x9A, 0x4C, 0x00, 0xF1])  # SEI; CLD; LDX #$FF; TXS; JMP $F100
+    put(0xF100, [0xA9, 0x02, 0x85, 0x00, 0xA9, 0x00, 0x85, 0x00,   # VSYNC on / off
+                 0xAD, 0x80, 0x02, 0x4A, 0x4A, 0x4A, 0x4A,         # LDA SWCHA; LSR x 4
+                 0x65, 0x90, 0x69, 0x07, 0x85, 0x90,               # ADC $90; ADC #7; STA $90
+                 0x29, 0x0F, 0xAA,                                 # AND #$0F; TAX
+                 0xB5, 0x00, 0x85, 0x80,                           # LDA $00,X; STA $80
+                 0xB5, 0x30, 0x85, 0x81,                           # LDA $30,X; STA $81
+                 0xBD, 0x80, 0x02, 0x85, 0x82,                     # LDA $0280,X; STA $82
+                 0xA5, 0x90, 0x85, 0x01,                           # LDA $90; STA VBLANK
+                 0x9D, 0x94, 0x02,                                 # STA $0294,X
+                 0xA0, 0x20,                                       # LDY #$20
+                 0x85, 0x02,                                       # STA WSYNC
+                 0xBD, 0x84, 0x02, 0x45, 0x83, 0x85, 0x83,         # LDA $0284,X; EOR $83; STA $83
+                 0xB5, 0x08, 0x45, 0x84, 0x85, 0x84,               # LDA $08,X; EOR $84; STA $84
+                 0x88, 0xD0, 0xEE,                                 # DEY; BNE (to STA WSYNC)
+                 0x4C, 0x00, 0xF1])                                # JMP $F100
+    rom[0xFFC], rom[0xFFD] = 0x00, 0xF0
+    return bytes(rom)

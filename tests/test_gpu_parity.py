@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from gpu_util import assert_same
-from romkit import branch_hotspot_rom, jsr_rom, stack_rom
+from romkit import branch_hotspot_rom, io_rom, jsr_rom, stack_rom
 
 pytestmark = pytest.mark.gpu
 
@@ -395,3 +395,23 @@ def test_branch_dummy_reads_switch_banks(fa, orc, sorted_):
         assert_same(gpu, ref, f"frame {f}")
     ram = ref.states()[0, 8:12]
     assert ram[0] > 0 and ram[2] > 0 and ram[1] == 0 and ram[3] == 0, ram
+
+
+@pytest.mark.parametrize("sorted_", [False, True])
+def test_tia_and_riot_registers_at_per_env_addresses(fa, orc, W, sorted_):
+    """Every TIA read register (collision latches, paddles against the
+    counter, INPT4/5, the constant ones, through two mirrors) and every RIOT
+    register (ports, DDRs, INTIM, INSTAT and mirrors; timer writes of every
+    interval and port / DDR writes) at a per-env index, with per-env VBLANK
+    paddle dumps: the per-lane data read's single load and register fix-up
+    and the RIOT paths, every frame equal to the oracle."""
+    rom = io_rom()
+    n, frames = 512, 64
+    gpu, ref = _pair(fa, orc, rom, n, sorted_)
+    acts = W.ActionStream(0x10A0, n)
+    for f in range(frames):
+        a = acts.next()
+        obs = gpu.step(a).reshape(n, 128)
+        ref.run(a[None, :])
+        assert np.array_equal(obs, ref.obs()), f"obs frame {f}"
+        assert_same(gpu, ref, f"frame {f}")
