@@ -452,15 +452,20 @@ struct Emu {
         st8(ram ? ram_off(a) : junk, v);
         if (!rom && !ram) io_write(a, v);
     }
-    // Zero page and stack page: bit 7 selects RAM, otherwise the TIA.
+    // Zero page and stack page: bit 7 selects RAM, otherwise the TIA (under a
+    // ballot: the stack of JSR / RTS / RTI / BRK rarely reaches $0100-$017F).
     __device__ __forceinline__ uint32_t zp_read(uint32_t a8) {
         uint32_t v = ld8(ram_off(a8));
-        if (!(a8 & 0x80u)) v = tia_read_lanes(a8);
+        if (__builtin_expect(__ballot(!(a8 & 0x80u)) != 0, 0)) {
+            if (!(a8 & 0x80u)) v = tia_read_lanes(a8);
+        }
         return v;
     }
     __device__ __forceinline__ void zp_write(uint32_t a8, uint32_t v) {
         st8((a8 & 0x80u) ? ram_off(a8) : junk, v);
-        if (!(a8 & 0x80u)) tia_write_lanes(a8, v);
+        if (__builtin_expect(__ballot(!(a8 & 0x80u)) != 0, 0)) {
+            if (!(a8 & 0x80u)) tia_write_lanes(a8, v);
+        }
     }
     __device__ __forceinline__ void push(uint32_t v) { zp_write(sp, v); sp = (sp - 1u) & 0xFFu; }
     __device__ __forceinline__ uint32_t pull() { sp = (sp + 1u) & 0xFFu; return zp_read(sp); }
