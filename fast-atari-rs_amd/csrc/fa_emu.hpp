@@ -1033,7 +1033,7 @@ struct Emu {
         if (!done) {
             uint32_t gb1 = b1, gb2 = b2, gop = op;
             uint4 gd = ld128(DESC0 + op * 16u);
-            if (!fz) fetch(gb1, gb2, gd, gop);
+            if (__builtin_expect(!fz, 0)) fetch(gb1, gb2, gd, gop);
             pc = (pc0 + ((gd.x >> D_LEN_SHIFT) & 3u)) & 0xFFFFu;
             cyc = execute(gb1, gb2, gd, fz);
         }
