@@ -950,35 +950,39 @@ struct Emu {
     // the current PC and bank.  Returns the cycle count, or 0 for an illegal
     // opcode.  Lanes whose PC is inside the ROM window (below the hotspot /
     // vector area) use those bytes, as fetching there has no side effects;
-    // the others fetch through the bus.  When the active lanes agree on the
-    // opcode, the wave takes the convergent path: a scalar dispatch on the
-    // opcode into the fast path specialised with that opcode's compile-time
-    // descriptor.
+    // the others fetch through the bus.
+    // Without GROUP every wave takes the general path (execute()): waves whose
+    // lanes agree on their control state run in the wave-uniform mode
+    // instead, so a per-opcode convergent path here would rarely run, and its
+    // merge with the general path costs every divergent instruction register
+    // copies and a long back edge (measured: branchy ROM 17.2 -> 16.1 ms per
+    // frame, F8 0.90 -> 0.86 ms without it).
     // With GROUP (set_sorted), a wave whose lanes hold at most two distinct
-    // opcodes runs each opcode group on the convergent path in turn (64-bit
-    // ballot + readlane find the groups); with three or more, the lanes share
-    // one pass of the general path, which costs less than one convergent pass
-    // per opcode.  Anything the fast path declines -- and divergent waves
-    // without GROUP -- goes through the general path (execute()), which
-    // replays the reference's bus accesses one by one.
+    // opcodes runs each opcode group on the convergent path in turn: a scalar
+    // dispatch on the group's opcode into the fast path specialised with that
+    // opcode's compile-time descriptor (64-bit ballot + readlane find the
+    // groups); with three or more, the lanes share one pass of the general
+    // path, which costs less than one convergent pass per opcode.  Anything
+    // the fast path declines goes through the general path, which replays
+    // the reference's bus accesses one by one.
     template <bool GROUP>
     __device__ __forceinline__ uint32_t exec(uint32_t op, uint32_t b1, uint32_t b2) {
         const uint32_t pc0 = pc;
         const bool fz = (pc0 & 0x1000u) && ((pc0 & 0xFFFu) < 0xFF0u);
-        const uint32_t key = fz ? op : 0x100u;
         uint32_t cyc = 0;
         bool done = false;
-        const uint64_t all = __ballot(1);
-        // The lane tests compare against an opaque copy of the group's key:
-        // were it the key itself, the optimiser would substitute the per-lane
-        // key for it where the two are equal, and the dispatch below would
-        // become a VALU compare tree under exec masks instead of scalar
-        // branches.
-        const uint32_t k1 = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)__builtin_ctzll(all));
-        uint32_t k1c = k1;
-        __asm__ volatile("" : "+s"(k1c));
-        const uint64_t g1 = __ballot(key == k1c);
         if constexpr (GROUP) {
+            const uint32_t key = fz ? op : 0x100u;
+            const uint64_t all = __ballot(1);
+            // The lane tests compare against an opaque copy of the group's key:
+            // were it the key itself, the optimiser would substitute the
+            // per-lane key for it where the two are equal, and the dispatch
+            // below would become a VALU compare tree under exec masks instead
+            // of scalar branches.
+            const uint32_t k1 = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)__builtin_ctzll(all));
+            uint32_t k1c = k1;
+            __asm__ volatile("" : "+s"(k1c));
+            const uint64_t g1 = __ballot(key == k1c);
             const uint64_t r2 = all & ~g1;
             uint32_t k2 = k1;
             uint64_t g2 = 0;
@@ -1009,24 +1013,6 @@ struct Emu {
                         default: break;
                         }
                     }
-                }
-            }
-        } else {
-            // The probability hint lets block placement keep the per-lane
-            // loop contiguous below the switch (short branches back to the
-            // loop top); divergent waves gain ~2 %, converged ones ~0.5 %.
-            const bool fast = k1 < 0x100u && g1 == all;
-            if (__builtin_expect_with_probability(fast, 1, 0.3)) {
-                switch (k1) {
-#define FA_CASE(OPC)                                                                          \
-                case OPC: {                                                                   \
-                    constexpr uint64_t D = descriptor_of(OPC);                                \
-                    done = fast_uniform((uint32_t)D, (uint32_t)(D >> 32), b1, b2, pc0, cyc);  \
-                    break;                                                                    \
-                }
-                    FA_FOR_EACH_OPCODE(FA_CASE)
-#undef FA_CASE
-                default: break;
                 }
             }
         }
