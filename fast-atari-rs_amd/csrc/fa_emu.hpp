@@ -203,7 +203,15 @@ struct Emu {
     }
 
     // ------------------------------------------------------------ RIOT
-    __device__ __forceinline__ void pia_tick(uint32_t n) { timer_tick(*this, n); }
+    // Pia::tick_n per lane: the prescaler step inline, the rest under a ballot
+    // (in a divergent wave some lane needs it in a few percent of the steps)
+    __device__ __forceinline__ void pia_tick(uint32_t n) {
+        const bool fast = !(tval == 0u && tuf) && tps > n;
+        tps -= fast ? n : 0u;
+        if (__builtin_expect(__ballot(!fast) != 0, 0)) {
+            if (!fast) timer_tick_slow(*this, n);
+        }
+    }
 
     // Pia::read for I/O addresses (src/pia.rs:169-199)
     __device__ __forceinline__ uint32_t pia_read(uint32_t addr) {

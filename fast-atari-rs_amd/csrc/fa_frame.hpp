@@ -22,11 +22,23 @@ namespace fa {
 // instruction and everything else.  Before each per-lane instruction a cheap
 // test (PC and TIA clock) decides whether the full test is worth making; a
 // failed full test backs off for a growing number of instructions.
+//
+// The reference's loop tests, before each instruction, whether VSYNC still
+// agrees with the loop (loop A while it is held, loop B until it rises) and
+// whether WSYNC is set, and after it the cycle cap.  Here one test covers
+// both rare conditions before the instruction and one the cap and an illegal
+// opcode after it: neither VSYNC nor WSYNC changes but by a TIA write, so the
+// common step runs no exec-mask branching of its own.  A WSYNC skip is
+// followed by the next instruction in the same iteration, which is the
+// reference's next step once its caps are checked (the skip leaves VSYNC as
+// it was and WSYNC clear).
 template <bool SORTED>
 __device__ __forceinline__ uint32_t run_frame(Emu &e, uint32_t &st, uint32_t &cpu_cyc, uint32_t &instr) {
     e.fcomp = 0;
     uint32_t cycles = 0;
     bool in_a = (e.vsync & 2u) != 0u;
+    uint32_t cap = in_a ? LOOPA_CAP : LOOPB_CAP;
+    uint32_t ina2 = in_a ? 2u : 0u;      // in_a as VSYNC's bit 1
     uint32_t hold = 0, backoff = 0;      // wave-uniform
     uint32_t op, b1, b2;                 // ROM bytes at PC, read ahead
     e.fetch_ahead(op, b1, b2);
@@ -39,6 +51,8 @@ __device__ __forceinline__ uint32_t run_frame(Emu &e, uint32_t &st, uint32_t &cp
                     const bool moved = u.cycles != u.c0;
                     u_leave(u, e, cycles, in_a, cpu_cyc, instr);
                     if (r == U_DONE) break;
+                    cap = in_a ? LOOPA_CAP : LOOPB_CAP;
+                    ina2 = in_a ? 2u : 0u;
                     e.fetch_ahead(op, b1, b2);
                     // declined at once: back off as after a failed test
                     backoff = moved ? 0u : (backoff ? (backoff < 64u ? 2u * backoff : 64u) : 1u);
@@ -51,31 +65,44 @@ __device__ __forceinline__ uint32_t run_frame(Emu &e, uint32_t &st, uint32_t &cp
         } else {
             --hold;
         }
-        // loop A while VSYNC is held, loop B until it rises
-        const bool v2 = (e.vsync & 2u) != 0u;
-        in_a = in_a & v2;
-        if (!in_a & v2) break;
-        bool stop = false;
-        if (__builtin_expect(e.wsync != 0u, 0)) {
-            uint32_t s = e.tia_skip();
-            if (s == 0u) { st = ENV_VSYNC_HELD; break; }   // only a clock >= 228 could stall here
-            e.pia_tick(s);
-            cycles += s;
-        } else {
-            // an illegal opcode (0 cycles) changes nothing below and ends the frame
-            uint32_t c = e.exec<SORTED>(op, b1, b2);
-            e.fetch_ahead(op, b1, b2);
-            cycles += c;
-            cpu_cyc += c;
-            stop = c == 0u;
-            instr += stop ? 0u : 1u;
-            st = stop ? ENV_ILLEGAL : st;
-            e.tia_tick(c);
-            e.pia_tick(c);
+        // one test for two rare events: WSYNC, and VSYNC disagreeing with
+        // the loop (loop A runs while VSYNC is held, loop B until it rises)
+        const uint32_t v2 = e.vsync & 2u;
+        if (__builtin_expect((e.wsync | (v2 ^ ina2)) != 0u, 0)) {
+            if (!in_a & (v2 != 0u)) break;             // VSYNC rose in loop B: the frame ends
+            if (in_a & (v2 == 0u)) {                    // VSYNC dropped: loop B from here
+                in_a = false;
+                ina2 = 0u;
+                cap = LOOPB_CAP;
+            }
+            if (e.wsync != 0u) {
+                // the skip, then the caps; VSYNC is unchanged and WSYNC clear,
+                // so the loop's next step is the instruction below
+                const uint32_t s = e.tia_skip();
+                if (s == 0u) { st = ENV_VSYNC_HELD; break; }   // only a clock >= 228 could stall here
+                e.pia_tick(s);
+                cycles += s;
+                if (cycles > cap) {
+                    st = in_a ? ENV_VSYNC_HELD : st;
+                    break;
+                }
+            }
         }
-        const bool capa = in_a & (cycles > LOOPA_CAP);
-        st = capa ? ENV_VSYNC_HELD : st;
-        if (stop | capa | (!in_a & (cycles > LOOPB_CAP))) break;
+        const uint32_t c = e.exec<SORTED>(op, b1, b2);
+        e.fetch_ahead(op, b1, b2);
+        cycles += c;
+        cpu_cyc += c;
+        instr += 1u;
+        e.tia_tick(c);
+        e.pia_tick(c);
+        if (__builtin_expect((c == 0u) | (cycles > cap), 0)) {
+            // an illegal opcode (0 cycles) changes nothing above and ends the
+            // frame; loop A's cap leaves VSYNC held
+            instr -= c == 0u ? 1u : 0u;
+            st = c == 0u ? ENV_ILLEGAL : st;
+            st = in_a & (cycles > cap) ? ENV_VSYNC_HELD : st;
+            break;
+        }
     }
     return cycles;
 }
