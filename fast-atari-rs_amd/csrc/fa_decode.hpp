@@ -3,7 +3,8 @@
 // Every documented NMOS 6502 opcode (151) is described by two 32-bit words
 // that steer one shared fetch / address / read / compute / write pipeline
 // with predicated selects instead of a per-opcode switch, plus two words with
-// the same decode as one-hot masks for the per-lane path (16 bytes an opcode).  Cycle counts and
+// the same decode as one-hot masks for the per-lane path and, in the table
+// the kernels stage, four words of register selectors (32 bytes an opcode).  Cycle counts and
 // the order of bus accesses are those of the reference's Cpu::step
 // (src/cpu.rs:245-1146); the table itself is derived from the instruction
 // set (mnemonic x addressing mode), not from the reference's code.
@@ -48,6 +49,7 @@ enum : uint32_t {
     D1_RMW_SHIFT = 6,       // 3 bits: 0 ASL, 1 ROL, 2 LSR, 3 ROR, 4 DEC, 5 INC
     D1_DELTA_SHIFT = 9,     // 2 bits: 0 none, 1 +1, 2 -1
     D1_SPEC_SHIFT = 11,     // 4 bits: special id
+    D1_SLOW = 1u << 15,     // descriptor table only: JSR, RTS, RTI, BRK, JMP (ind) (the per-lane slow pass)
     D1_FMASK_SHIFT = 16,    // 8 bits: flags written (K_FLAGS) / tested (branch)
     D1_FVAL_SHIFT = 24,     // 8 bits: value written / branch taken when set
 };
@@ -90,7 +92,18 @@ enum : uint32_t {
     H_PULL = 30,            // PLA / PLP: SP + 1, the result read from $0100 + SP
     H_PREG = 31,            // PHP / PLP: the status register is the source / destination
 };
-constexpr uint32_t kDescWords = 4;                  // words per opcode
+// ---- d4 .. d7 (descriptor table only): the per-lane stage keeps A, X, Y and
+// SP as the bytes of one word (A | X << 8 | Y << 16 | SP << 24) for the
+// step, and selects from it with v_perm_b32 (byte selectors 0-3: a register,
+// 4: the result, 0x0C: zero):
+//   d4  the source register into byte 0 (none: 0)
+//   d5  byte 0: the index added to the address (X / Y / none), byte 1: the
+//       index added to a (zp,X) pointer address (X / none)
+//   d6  the result into the destination register's byte, the others kept
+//   d7  added to the word before d6: SP + 1 (pulls) or SP - 1 (pushes) in the
+//       top byte, which wraps within the byte
+// All four follow from d2 / d3 (build_descriptors).
+constexpr uint32_t kDescWords = 8;                  // words per opcode
 constexpr uint32_t kDescBytes = 256u * kDescWords * 4u;
 
 // d2 and d3 of an opcode from its d0 and d1.
@@ -341,17 +354,34 @@ constexpr uint64_t descriptor_of(uint32_t code) {
 // opcodes are CLS_ILLEGAL with length 1.
 inline void build_descriptors(uint32_t *table) {
     for (int i = 0; i < 256; ++i) {
-        table[4 * i] = (CLS_ILLEGAL << D_CLS_SHIFT) | (1u << D_LEN_SHIFT);
-        table[4 * i + 1] = table[4 * i + 2] = table[4 * i + 3] = 0;
+        uint32_t *t = table + kDescWords * i;
+        t[0] = (CLS_ILLEGAL << D_CLS_SHIFT) | (1u << D_LEN_SHIFT);
+        t[1] = t[2] = t[3] = 0;
+        t[4] = t[5] = 0x0C0C0C0Cu;
+        t[6] = 0x03020100u;
+        t[7] = 0;
     }
     for (const detail::Op &o : detail::kOps) {
         uint32_t d0 = 0, d1 = 0, d2 = 0, d3 = 0;
         detail::describe(o, d0, d1);
         describe_lanes(d0, d1, d2, d3);
-        table[4 * o.code] = d0;
-        table[4 * o.code + 1] = d1;
-        table[4 * o.code + 2] = d2;
-        table[4 * o.code + 3] = d3;
+        uint32_t *t = table + kDescWords * o.code;
+        t[0] = d0;
+        t[1] = d1 | (((d0 & 3u) == CLS_SPECIAL && d3 == 0u) ? D1_SLOW : 0u);
+        t[2] = d2;
+        t[3] = d3;
+        // d4 .. d7: v_perm selectors over the register word
+        uint32_t ss = 0x0C0C0C0Cu, sd = 0x03020100u;
+        for (uint32_t k = 0; k < 4u; ++k) {
+            if (d2 & (1u << (G_SRC + k))) ss = 0x0C0C0C00u | k;
+            if (d2 & (1u << (G_DST + k))) sd = (sd & ~(0xFFu << (8u * k))) | (4u << (8u * k));
+        }
+        const uint32_t ie = (d2 & (1u << G_IX_X)) ? 1u : ((d2 & (1u << G_IX_Y)) ? 2u : 0x0Cu);
+        const uint32_t iz = (d2 & (1u << G_IZX)) ? 1u : 0x0Cu;
+        t[4] = ss;
+        t[5] = 0x0C0C0000u | (iz << 8) | ie;
+        t[6] = sd;
+        t[7] = (d3 & (1u << H_PULL)) ? 0x01000000u : ((d3 & (1u << H_PUSH)) ? 0xFF000000u : 0u);
     }
 }
 

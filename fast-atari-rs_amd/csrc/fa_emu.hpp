@@ -39,7 +39,7 @@ constexpr uint32_t DESC0 = 0;        // LDS offset of the descriptor table
 constexpr uint32_t ROM0 = kDescBytes; // LDS offset of the ROM image
 // A byte that always reads 0: the second word of an undocumented opcode's
 // descriptor ($02).
-constexpr uint32_t ZERO_BYTE = DESC0 + 0x02u * 16u + 4u;
+constexpr uint32_t ZERO_BYTE = DESC0 + 0x02u * 32u + 4u;
 
 // Byte offsets inside the 222-byte AtariStateGpu (src/cuda_env.rs:12-38).
 enum : uint32_t {
@@ -485,16 +485,18 @@ struct Emu {
     // reference's order (opcode, then operands or the phantom read of PC).
     // JSR's high operand byte is read here for its side effects (an INTIM
     // read, a hotspot) and read again after the two pushes (execute_slow()).
-    __device__ __forceinline__ void fetch(uint32_t &b1, uint32_t &b2, uint4 &d, uint32_t &op) {
+    __device__ __forceinline__ void fetch(uint32_t &b1, uint32_t &b2, uint4 &d, uint4 &sl, uint32_t &op) {
         if ((pc & 0x1000u) && (pc & 0xFFFu) < 0xFF0u) {
             const uint32_t o = ROM0 + (bank << 12) + (pc & 0xFFFu);
             op = ld8(o);
             b1 = ld8(o + 1u);
             b2 = ld8(o + 2u);
-            d = ld128(DESC0 + op * 16u);
+            d = ld128(DESC0 + op * 32u);
+            sl = ld128(DESC0 + op * 32u + 16u);
         } else {
             op = bus_read(pc);
-            d = ld128(DESC0 + op * 16u);
+            d = ld128(DESC0 + op * 32u);
+            sl = ld128(DESC0 + op * 32u + 16u);
             const uint32_t d0 = d.x;
             const uint32_t len = (d0 >> D_LEN_SHIFT) & 3u;
             const bool illegal = (d0 & 3u) == CLS_ILLEGAL;
@@ -552,7 +554,8 @@ struct Emu {
     }
 
     // One instruction of every lane whose PC lies in the cartridge window, in
-    // one straight-line pass steered by the one-hot words d2 / d3: loads,
+    // one straight-line pass steered by the one-hot words d2 / d3 and the
+    // register selectors d4 .. d7 (A, X, Y, SP as one word for the step): loads,
     // stores, ALU, read-modify-write, transfers, flag instructions,
     // conditional branches, JMP, PHA / PHP / PLA / PLP.  Only TIA / RIOT
     // writes, RIOT reads, zero-page pointers in the TIA and decimal mode
@@ -569,16 +572,17 @@ struct Emu {
     // the window, whose H_BR is masked off by fz) change nothing here;
     // illegal opcodes return 0 cycles.
     __device__ __forceinline__ uint32_t compute_lanes(uint32_t b1, uint32_t b2, uint32_t d0, uint32_t d1, uint32_t g,
-                                                      uint32_t h, bool fz) {
+                                                      uint32_t h, const uint4 &sl, bool fz) {
         uint32_t cyc = (d0 >> D_CYC_SHIFT) & 7u;
         const uint32_t mbr = bm(h, H_BR) & mk(fz);
         const uint32_t mpush = bm(h, H_PUSH), mpull = bm(h, H_PULL), mpreg = bm(h, H_PREG);
-        const uint32_t s = (a & bm(g, G_SRC)) | (x & bm(g, G_SRC + 1u)) | (y & bm(g, G_SRC + 2u)) |
-                           (sp & bm(g, G_SRC + 3u)) | ((p | FB | FU) & mpreg & mpush);
+        const uint32_t regs = a | (x << 8) | (y << 16) | (sp << 24);
+        const uint32_t s = __builtin_amdgcn_perm(0u, regs, sl.x) | ((p | FB | FU) & mpreg & mpush);
+        const uint32_t ix = __builtin_amdgcn_perm(0u, regs, sl.y);   // byte 0: the address's index; byte 1: (zp,X)'s X
         // effective address (every lane computes one; only lanes with a data access use it)
         const uint32_t zpm = bm(d0, 8u);                         // D_ZP: 8-bit address arithmetic
         const bool ind = (g & (1u << G_IND)) != 0u;
-        const uint32_t zp = (b1 + (x & bm(g, G_IZX))) & 0xFFu;
+        const uint32_t zp = (b1 + (ix >> 8)) & 0xFFu;
         const uint32_t zq = (zp + 1u) & 0xFFu;
         uint32_t lo = ld8(ram_off(zp)), hi = ld8(ram_off(zq));
         if (__builtin_expect(__ballot(ind & (((zp & zq) & 0x80u) == 0u)) != 0, 0)) {
@@ -592,14 +596,13 @@ struct Emu {
         const uint32_t jbase = base;
         base = (pc & mbr) | (base & ~mbr);
         const uint32_t rel = (b1 - ((b1 & 0x80u) << 1)) & mbr;
-        const uint32_t ea = (base + ((x & bm(g, G_IX_X)) | (y & bm(g, G_IX_Y)) | rel)) & (0xFFFFu ^ (0xFF00u & zpm));
+        const uint32_t ea = (base + ((ix & 0xFFu) | rel)) & (0xFFFFu ^ (0xFF00u & zpm));
         const bool taken = (mbr != 0u) & (((p & (d1 >> D1_FMASK_SHIFT) & 0xFFu) != 0u) == ((d1 >> D1_FVAL_SHIFT) != 0u));
         const bool cross = ((base ^ ea) & 0xFF00u) != 0u;
         const bool cross_fix = (((d0 & D_DUMMY_CROSS) != 0u) | taken) & cross;
         touch_if(((d0 & D_DUMMY_ALWAYS) != 0u) | cross_fix, (base & 0xFF00u) | (ea & 0xFFu));
         cyc += (cross_fix ? 1u : 0u) + (taken ? 1u : 0u);
         pc = pick(taken, ea, pick((h & (1u << H_JMP)) != 0u, jbase, pc));
-        sp = (sp + (1u & mpull) - (1u & mpush)) & 0xFFu;
         // operand: immediate, register or the data read.  The read is one
         // load from the ROM window, a RAM mirror, the TIA register's image
         // byte or a zero byte (the flat array's padding); the RIOT branches.
@@ -643,11 +646,11 @@ struct Emu {
         const uint32_t cf = (((sum >> 8) & bm(g, G_C_SUM)) | ((v >> 7) & bm(g, G_C_B7)) | (v & bm(g, G_C_B0))) & FC;
         const uint32_t cand = (nv & FN) | vf | (r == 0u ? FZ : 0u) | cf;
         p = (p & ~(h & 0xFFu)) | (cand & (h >> H_TAKE)) | ((h >> H_SET) & 0xFFu) | (v & ~(FB | FU) & mpreg & mpull);
-        const uint32_t ma = bm(g, G_DST), mx = bm(g, G_DST + 1u), my = bm(g, G_DST + 2u), ms = bm(g, G_DST + 3u);
-        a = (r & ma) | (a & ~ma);
-        x = (r & mx) | (x & ~mx);
-        y = (r & my) | (y & ~my);
-        sp = (r & ms) | (sp & ~ms);
+        const uint32_t rg = __builtin_amdgcn_perm(r, regs + sl.w, sl.z);
+        a = rg & 0xFFu;
+        x = (rg >> 8) & 0xFFu;
+        y = (rg >> 16) & 0xFFu;
+        sp = rg >> 24;
         const bool dec = ((d0 & D_DEC) != 0u) & ((p0 & FD) != 0u);
         if (__builtin_expect(__ballot(dec) != 0, 0)) decimal_lanes(dec, (d0 & D_INV) != 0u, a0, v, p0);
         // write-back
@@ -678,11 +681,10 @@ struct Emu {
 
     // Executes one decoded instruction (pc already points past it).  Returns
     // the cycle count, or 0 for an illegal opcode (src/cpu.rs:1139-1141).
-    __device__ __forceinline__ uint32_t execute(uint32_t b1, uint32_t b2, const uint4 &d, bool fz) {
-        const uint32_t cls = d.x & 3u;
+    __device__ __forceinline__ uint32_t execute(uint32_t b1, uint32_t b2, const uint4 &d, const uint4 &sl, bool fz) {
         // JSR, RTS, RTI, BRK, JMP (ind), and branches outside the window
-        const bool slow = ((cls == CLS_SPECIAL) & ((d.w >> H_JMP) == 0u)) | ((cls == CLS_BRANCH) & !fz);
-        uint32_t cyc = compute_lanes(b1, b2, d.x, d.y, d.z, d.w, fz);
+        const bool slow = ((d.y & D1_SLOW) != 0u) | (((d.w >> H_BR) & 1u) > (uint32_t)fz);
+        uint32_t cyc = compute_lanes(b1, b2, d.x, d.y, d.z, d.w, sl, fz);
         if (__builtin_expect(__ballot(slow) != 0, 0)) {
             if (slow) cyc = execute_slow(b1, b2, d);
         }
@@ -1026,10 +1028,11 @@ struct Emu {
         }
         if (!done) {
             uint32_t gb1 = b1, gb2 = b2, gop = op;
-            uint4 gd = ld128(DESC0 + op * 16u);
-            if (__builtin_expect(!fz, 0)) fetch(gb1, gb2, gd, gop);
+            uint4 gd = ld128(DESC0 + op * 32u);
+            uint4 gs = ld128(DESC0 + op * 32u + 16u);
+            if (__builtin_expect(!fz, 0)) fetch(gb1, gb2, gd, gs, gop);
             pc = (pc0 + ((gd.x >> D_LEN_SHIFT) & 3u)) & 0xFFFFu;
-            cyc = execute(gb1, gb2, gd, fz);
+            cyc = execute(gb1, gb2, gd, gs, fz);
         }
         return cyc;
     }

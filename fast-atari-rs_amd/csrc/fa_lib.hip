@@ -367,7 +367,7 @@ size_t fa_num_envs(const fa_env *e) { return e ? e->n : 0; }
 
 int fa_descriptor_table(uint32_t *out, size_t n_words) {
     if (!out) return fail(FA_EINVAL, "out is NULL");
-    if (n_words != fa::kDescBytes / 4) return fail(FA_EINVAL, "n_words must be 1024");
+    if (n_words != fa::kDescBytes / 4) return fail(FA_EINVAL, "n_words must be 2048");
     fa::descriptor_table(out);
     return FA_OK;
 }

@@ -148,9 +148,10 @@ int fa_set_block_size(fa_env *env, uint32_t block);
 int fa_get_block_size(const fa_env *env, uint32_t *block);
 /* LDS bytes available to one workgroup on the handle's device. */
 int fa_get_max_lds(const fa_env *env, size_t *bytes);
-/* The per-opcode descriptor table the kernels stage into LDS: 256 opcodes x 4
- * words (the decode d0 / d1 and the same decode as one-hot masks d2 / d3, see
- * csrc/fa_decode.hpp); n_words must be 1024.  Host only (no GPU needed). */
+/* The per-opcode descriptor table the kernels stage into LDS: 256 opcodes x 8
+ * words (the decode d0 / d1, the same decode as one-hot masks d2 / d3 and the
+ * per-lane register selectors d4 .. d7, see csrc/fa_decode.hpp); n_words must
+ * be 2048.  Host only (no GPU needed). */
 int fa_descriptor_table(uint32_t *out, size_t n_words);
 
 /* ---- heterogeneous batches (several ROM classes stepped together) ----
@@ -353,7 +354,7 @@ int fa_probe_valu_peak(uint32_t iters, uint32_t reps, double *ms_per_launch, uin
  * stream), returns without waiting.  Calls that overlap in time on one GPU
  * share the array and race; the first call on a device sets it up. */
 #define FA_NOALLOC_MAX_ENVS 65536
-#define FA_NOALLOC_SCRATCH_BYTES 16421120
+#define FA_NOALLOC_SCRATCH_BYTES 16425216
 int fa_step_noalloc(const uint8_t *rom, size_t rom_len, AtariStateGpu *d_states, uint32_t *d_status,
                     const uint8_t *d_actions, size_t n, uint32_t k, uint8_t *d_obs, void *stream);
 

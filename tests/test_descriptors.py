@@ -6,7 +6,12 @@ the one-hot words d2 / d3 select the source, the result, the destination
 register, the stack-pointer step and the status bits that the instruction's
 textbook definition gives, on random registers and operands (binary mode;
 decimal ADC / SBC is checked end-to-end by the GPU parity tests); branches and
-JMP carry their own d3 bit, the other stack / jump instructions none.
+JMP carry their own d3 bit, the other stack / jump instructions none.  The
+register selectors d4 .. d7 (v_perm byte selectors over A | X << 8 | Y << 16 |
+SP << 24) are what the per-lane stage uses for the source, index and
+destination registers and the stack-pointer step: the textbook check runs
+through them, the index selectors are checked against the addressing modes,
+and d1's slow-pass bit marks exactly JSR, RTS, RTI, BRK and JMP (ind).
 """
 import ctypes
 
@@ -28,6 +33,8 @@ G_BIT, G_C_SUM, G_C_B7, G_C_B0 = 28, 29, 30, 31
 H_TAKE, H_SET, H_M_AND, H_M_XOR, H_M_LD = 8, 16, 24, 25, 26
 H_BR, H_JMP, H_PUSH, H_PULL, H_PREG = 27, 28, 29, 30, 31
 PUSH_PULL = {"PHA", "PHP", "PLA", "PLP"}
+D1_SLOW = 1 << 15
+WORDS = 8
 FC, FZ, FI, FD, FV, FN = 0x01, 0x02, 0x04, 0x08, 0x40, 0x80
 
 
@@ -35,10 +42,22 @@ FC, FZ, FI, FD, FV, FN = 0x01, 0x02, 0x04, 0x08, 0x40, 0x80
 def table():
     import fastatari
     L = fastatari.load_library()
-    t = np.zeros(1024, np.uint32)
-    assert L.fa_descriptor_table(t.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), 1024) == 0
-    assert L.fa_descriptor_table(t.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), 512) == -1
-    return t.reshape(256, 4).astype(np.int64)
+    t = np.zeros(256 * WORDS, np.uint32)
+    assert L.fa_descriptor_table(t.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), 256 * WORDS) == 0
+    assert L.fa_descriptor_table(t.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), 1024) == -1
+    return t.reshape(256, WORDS).astype(np.int64)
+
+
+def perm(s0, s1, sel):
+    """v_perm_b32 for the selector bytes the table uses: 0-3 pick a byte of
+    s1, 4-7 a byte of s0, 0x0C gives 0."""
+    src = [(s1 >> (8 * k)) & 0xFF for k in range(4)] + [(s0 >> (8 * k)) & 0xFF for k in range(4)]
+    out = 0
+    for i in range(4):
+        b = (sel >> (8 * i)) & 0xFF
+        assert b < 8 or b == 0x0C, hex(sel)
+        out |= (src[b] if b < 8 else 0) << (8 * i)
+    return out
 
 
 def _bit(w, k):
@@ -48,12 +67,11 @@ def _bit(w, k):
 def lanes_select(d, a, x, y, sp, p, v_in):
     """The per-lane compute stage's selections, evaluated with the table's
     words: returns (a, x, y, sp, p, r)."""
-    d0, _, g, h = (int(w) for w in d)
+    d0, _, g, h, ssel, _, dsel, spstep = (int(w) for w in d)
     m = 0xFFFFFFFF
     push, pull, preg = _bit(h, H_PUSH), _bit(h, H_PULL), _bit(h, H_PREG)
-    s = ((a & _bit(g, G_SRC)) | (x & _bit(g, G_SRC + 1)) | (y & _bit(g, G_SRC + 2)) | (sp & _bit(g, G_SRC + 3))
-         | ((p | 0x30) & preg & push))
-    sp = (sp + (1 & pull) - (1 & push)) & 0xFF
+    regs = a | (x << 8) | (y << 16) | (sp << 24)
+    s = perm(0, regs, ssel) | ((p | 0x30) & preg & push)
     # D_RD or a pull: memory operand
     v = v_in if ((d0 & (1 << 14)) or pull) else (v_in if (d0 & D_IMM) else s)
     lg = (a & v & _bit(h, H_M_AND)) | ((a ^ v) & _bit(h, H_M_XOR)) | (v & _bit(h, H_M_LD))
@@ -72,11 +90,8 @@ def lanes_select(d, a, x, y, sp, p, v_in):
     cf = (((sm >> 8) & _bit(g, G_C_SUM)) | ((v >> 7) & _bit(g, G_C_B7)) | (v & _bit(g, G_C_B0))) & FC
     cand = (nv & FN) | vf | (FZ if r == 0 else 0) | cf
     p2 = ((p & ~(h & 0xFF)) | (cand & (h >> H_TAKE)) | ((h >> H_SET) & 0xFF) | (v & 0xCF & preg & pull)) & 0xFF
-    regs = []
-    for k, old in enumerate((a, x, y, sp)):
-        mk = _bit(g, G_DST + k)
-        regs.append(((r & mk) | (old & ~mk)) & 0xFF)
-    return (*regs, p2, r & 0xFF)
+    rg = perm(r & m, (regs + spstep) & m, dsel)
+    return (rg & 0xFF, (rg >> 8) & 0xFF, (rg >> 16) & 0xFF, rg >> 24, p2, r & 0xFF)
 
 
 def textbook(mn, mode, a, x, y, sp, p, v):
@@ -183,6 +198,31 @@ def test_addressing_words_match_the_modes(table):
                 "izx": {G_IND, G_IZX}, "izy": {G_IND, G_IX_Y}}.get(mode, set())
         got = {k for k in (G_IX_X, G_IX_Y, G_IZX, G_IND) if (g >> k) & 1}
         assert got == want, f"{mn} {mode}"
+
+
+def test_index_selectors_match_the_modes(table):
+    regs = 0x44332211                      # A = $11, X = $22, Y = $33, SP = $44
+    for op in range(256):
+        d4, d5, d6, d7 = (int(w) for w in table[op, 4:8])
+        if op not in OPCODES:
+            assert (d4, d5, d6, d7) == (0x0C0C0C0C, 0x0C0C0C0C, 0x03020100, 0), f"{op:#04x}"
+            continue
+        mn, mode, _ = OPCODES[op]
+        ix = perm(0, regs, d5)
+        if mn in BRANCH or mn in STACK_JUMP:
+            assert ix == 0, f"{mn} {mode}"
+            continue
+        want_ea = {"zpx": 0x22, "abx": 0x22, "zpy": 0x33, "aby": 0x33, "izy": 0x33}.get(mode, 0)
+        want_zp = 0x22 if mode == "izx" else 0
+        assert ix == want_ea | (want_zp << 8), f"{mn} {mode}: {ix:#x}"
+
+
+def test_slow_pass_bit(table):
+    for op in range(256):
+        d1 = int(table[op, 1])
+        mn, mode = OPCODES[op][:2] if op in OPCODES else (None, None)
+        slow = mn in ("JSR", "RTS", "RTI", "BRK") or (mn == "JMP" and mode == "ind")
+        assert bool(d1 & D1_SLOW) == slow, f"{op:#04x}"
 
 
 def test_branch_and_jump_words(table):
