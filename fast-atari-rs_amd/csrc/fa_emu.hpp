@@ -974,9 +974,10 @@ struct Emu {
     // groups); with three or more, the lanes share one pass of the general
     // path, which costs less than one convergent pass per opcode.  Anything
     // the fast path declines goes through the general path, which replays
-    // the reference's bus accesses one by one.
+    // the reference's bus accesses one by one.  `grouped` (GROUP only, may be
+    // null) reports whether the groups covered the wave.
     template <bool GROUP>
-    __device__ __forceinline__ uint32_t exec(uint32_t op, uint32_t b1, uint32_t b2) {
+    __device__ __forceinline__ uint32_t exec(uint32_t op, uint32_t b1, uint32_t b2, bool *grouped = nullptr) {
         const uint32_t pc0 = pc;
         const bool fz = (pc0 & 0x1000u) && ((pc0 & 0xFFFu) < 0xFF0u);
         uint32_t cyc = 0;
@@ -1002,6 +1003,7 @@ struct Emu {
                 __asm__ volatile("" : "+s"(k2c));
                 g2 = __ballot(key == k2c);
             }
+            if (grouped) *grouped = (g1 | g2) == all;
             if ((g1 | g2) == all) {
 #pragma nounroll
                 for (uint32_t q = 0; q < (r2 != 0 ? 2u : 1u); ++q) {

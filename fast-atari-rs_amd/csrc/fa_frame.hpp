@@ -32,6 +32,12 @@ namespace fa {
 // followed by the next instruction in the same iteration, which is the
 // reference's next step once its caps are checked (the skip leaves VSYNC as
 // it was and WSYNC clear).
+//
+// With SORTED (opcode grouping), a step whose wave holds three or more
+// distinct opcodes backs the grouping test off like the uniform-mode test
+// (1, 2, 4 .. 64 steps), and the steps in between run the plain kernel's
+// per-lane step itself, so a divergent phase pays neither the group test nor
+// the grouped dispatch's merge with the general path.
 template <bool SORTED>
 __device__ __forceinline__ uint32_t run_frame(Emu &e, uint32_t &st, uint32_t &cpu_cyc, uint32_t &instr) {
     e.fcomp = 0;
@@ -40,6 +46,7 @@ __device__ __forceinline__ uint32_t run_frame(Emu &e, uint32_t &st, uint32_t &cp
     uint32_t cap = in_a ? LOOPA_CAP : LOOPB_CAP;
     uint32_t ina2 = in_a ? 2u : 0u;      // in_a as VSYNC's bit 1
     uint32_t hold = 0, backoff = 0;      // wave-uniform
+    uint32_t ghold = 0, gback = 0;       // wave-uniform: grouping attempts (SORTED)
     uint32_t op, b1, b2;                 // ROM bytes at PC, read ahead
     e.fetch_ahead(op, b1, b2);
     while (true) {
@@ -88,7 +95,20 @@ __device__ __forceinline__ uint32_t run_frame(Emu &e, uint32_t &st, uint32_t &cp
                 }
             }
         }
-        const uint32_t c = e.exec<SORTED>(op, b1, b2);
+        uint32_t c;
+        if constexpr (SORTED) {
+            if (ghold == 0u) {
+                bool grouped = false;
+                c = e.exec<true>(op, b1, b2, &grouped);
+                gback = grouped ? 0u : (gback ? (gback < 64u ? 2u * gback : 64u) : 1u);
+                ghold = gback;
+            } else {
+                --ghold;
+                c = e.exec<false>(op, b1, b2);
+            }
+        } else {
+            c = e.exec<false>(op, b1, b2);
+        }
         e.fetch_ahead(op, b1, b2);
         cycles += c;
         cpu_cyc += c;
