@@ -203,15 +203,7 @@ struct Emu {
     }
 
     // ------------------------------------------------------------ RIOT
-    // Pia::tick_n per lane: the prescaler step inline, the rest under a ballot
-    // (in a divergent wave some lane needs it in a few percent of the steps)
-    __device__ __forceinline__ void pia_tick(uint32_t n) {
-        const bool fast = !(tval == 0u && tuf) && tps > n;
-        tps -= fast ? n : 0u;
-        if (__builtin_expect(__ballot(!fast) != 0, 0)) {
-            if (!fast) timer_tick_slow(*this, n);
-        }
-    }
+    __device__ __forceinline__ void pia_tick(uint32_t n) { timer_tick(*this, n); }
 
     // Pia::read for I/O addresses (src/pia.rs:169-199)
     __device__ __forceinline__ uint32_t pia_read(uint32_t addr) {
@@ -547,9 +539,11 @@ struct Emu {
         return pick(r4 < 8u, r4, __builtin_amdgcn_ubfe(0x009800BAu, (r4 & 7u) << 2, 4u));
     }
     // 8, 9: 0x80 once the paddle counter has reached the value; 10, 11: 0x80;
-    // 12, 13: 0x80 while the input latch is set; 14, 15: 0 (src/headless_tia.rs:205-223)
+    // 12, 13: 0x80 while the input latch is set; 14, 15: 0 (src/headless_tia.rs:205-223).
+    // pcount is a 16-bit count and m a byte, so pcount >= m is the sign bit of
+    // pcount - m cleared, and m != 0 is min(m, 1).
     __device__ __forceinline__ uint32_t tia_fix(uint32_t r4, uint32_t m) const {
-        const uint32_t bits = 2u | (pcount >= m ? 1u : 0u) | (m != 0u ? 4u : 0u);
+        const uint32_t bits = 2u | (((pcount - m) >> 31) ^ 1u) | (__builtin_elementwise_min(m, 1u) << 2);
         return pick(r4 < 8u, m, ((bits >> ((r4 >> 1) & 3u)) & 1u) << 7);
     }
 

@@ -578,17 +578,19 @@ __device__ __forceinline__ uint32_t u_run(UCtl &u, Emu &e, uint32_t &st) {
             // hotspot / vector area these fetches have no side effects
             if (__builtin_expect(((u.pc & 0x1FFFu) - 0x1000u) >= 0xFF0u, 0)) return U_EXIT;
             const uint32_t op = first_lane(f0), b1 = first_lane(f1), b2 = first_lane(f2);
+            // a handler that declines (0 cycles) leaves from its own case, so
+            // the common path after the dispatch carries no test for it
             uint32_t cyc = 0;
             switch (op) {
 #define FA_UCASE(OPC)                                                   \
             case OPC:                                                   \
                 cyc = u_op<descriptor_of(OPC)>(u, e, b1, b2);           \
+                if (__builtin_expect(cyc == 0u, 0)) return U_EXIT;      \
                 break;
                 FA_FOR_EACH_OPCODE(FA_UCASE)
 #undef FA_UCASE
-            default: break;
+            default: return U_EXIT;
             }
-            if (__builtin_expect(cyc == 0u, 0)) return U_EXIT;
             u.cycles += cyc;
             u.cpu += cyc;
             u.instr += 1u;
