@@ -151,6 +151,22 @@ __device__ __forceinline__ uint32_t u_tia_skip(UCtl &u, Emu &e) {
 // per-instruction test is one compare.
 __device__ __forceinline__ uint32_t u_tlim(const UCtl &u) { return (u.tval == 0u && u.tuf) ? 0u : u.tps; }
 
+// What the frame loop does after an instruction of `cyc` cycles: the counts,
+// the TIA clock, and Pia::tick_n (its fast branch inline, the rest out of line).
+__device__ __forceinline__ void u_step(UCtl &u, Emu &e, uint32_t cyc) {
+    u.cycles += cyc;
+    u.cpu += cyc;
+    u.instr += 1u;
+    u_tia_tick(u, e, cyc);
+    if (__builtin_expect(u.tlim > cyc, 1)) {
+        u.tps -= cyc;
+        u.tlim -= cyc;
+    } else {
+        timer_tick_slow(u, cyc);
+        u.tlim = u_tlim(u);
+    }
+}
+
 // ---------------------------------------------------------------- bus
 // Side effects of a read at a wave-uniform 13-bit address whose value is not
 // used: a hotspot switches the bank, INTIM clears the underflow flag.
@@ -379,6 +395,63 @@ __device__ __forceinline__ bool u_mem_uni(UCtl &u, Emu &e, uint32_t s, uint32_t 
     return true;
 }
 
+// A poll of the RIOT timer: the load at u.pc (BIT / LDA / LDX / LDY abs of
+// INTIM or INSTAT, the caller has checked the address) followed, at pcn, by a
+// branch back to it on a flag that the load sets from the value alone (N; V
+// after BIT; Z after LDA / LDX / LDY).  While the branch would be taken, whole
+// iterations -- the load, then the branch -- run here on the scalar state
+// with the steps the frame loop takes after each instruction: the value is
+// known before the read (INTIM: the timer value, INSTAT: the underflow flag),
+// the read's one side effect (INTIM clears the underflow flag) is made, and
+// the flags and register the load writes per lane are written again by the
+// load that follows.  Iterations end an iteration short of the frame loop's
+// cap tests, so those tests fail on every instruction run here as they would
+// in the loop.  Returns with u.pc still at the load, which the caller then
+// runs as usual.
+template <bool BIT>
+__device__ __forceinline__ void u_poll(UCtl &u, Emu &e, uint32_t a13, uint32_t pcn) {
+    constexpr uint32_t kLoad = 4u, kBranch = 3u;          // abs load; branch taken, same page
+    if (((pcn & 0x1FFFu) - 0x1000u) >= 0xFF0u) return;    // the branch's fetch bound (u_run)
+    if ((u.pc ^ (pcn + 2u)) & 0xFF00u) return;            // the taken branch crosses a page
+    const uint32_t o = ROM0 + (u.bank << 12) + (pcn & 0xFFFu);
+    const uint32_t bop = first_lane(ld8(o)), boff = first_lane(ld8(o + 1u));
+    if ((bop & 0x1Fu) != 0x10u || boff != 0xFBu) return;   // not a branch back to the load
+    // branch opcodes: bits 7-6 the flag (N, V, C, Z), bit 5 the value taken on
+    const uint32_t fl = bop >> 6;
+    if (fl == 2u || fl == (BIT ? 3u : 1u)) return;         // C; Z after BIT, V after a load: per lane
+    // taken when ((v & m) != 0) == want
+    const uint32_t m = fl == 0u ? 0x80u : (fl == 1u ? 0x40u : 0xFFu);
+    const uint32_t want = ((bop >> 5) & 1u) ^ (fl == 3u ? 1u : 0u);
+    const bool intim = (a13 & 1u) == 0u;
+    constexpr uint32_t kIter = kLoad + kBranch, kLimit = LOOPB_CAP - kIterCycles;
+    while (u.cycles <= kLimit - kIter) {
+        const uint32_t v = intim ? (u.tval & 0xFFu) : (u.tuf ? 0x80u : 0u);
+        if (((v & m) != 0u ? 1u : 0u) != want) return;
+        if (intim) {
+            u.tuf = 0;
+            u.tlim = u.tps;
+        }
+        // k iterations that end no line (clock + 21 k <= 227), take the
+        // timer's fast path at every instruction (tlim > 7 k) and stay below
+        // the limit change only the counts, the TIA clock and the prescaler
+        // count, and leave the value read (and so the branch) as it is
+        uint32_t k = (227u - u.clock) / (3u * kIter);
+        k = __builtin_elementwise_min(k, (u.tlim - 1u) / kIter);
+        k = __builtin_elementwise_min(k, (kLimit - u.cycles) / kIter);
+        if (u.tlim != 0u && k != 0u) {
+            u.cycles += kIter * k;
+            u.cpu += kIter * k;
+            u.instr += 2u * k;
+            u.clock += 3u * kIter * k;
+            u.tps -= kIter * k;
+            u.tlim -= kIter * k;
+            continue;
+        }
+        u_step(u, e, kLoad);
+        u_step(u, e, kBranch);
+    }
+}
+
 template <uint32_t D0, uint32_t D1>
 __device__ __forceinline__ bool u_compute(UCtl &u, Emu &e, uint32_t b1, uint32_t b2, uint32_t pcn, uint32_t &cyc) {
     constexpr uint32_t srcsel = (D1 >> D1_SRC_SHIFT) & 7u;
@@ -408,6 +481,13 @@ __device__ __forceinline__ bool u_compute(UCtl &u, Emu &e, uint32_t b1, uint32_t
                     const uint32_t ea = (base + idx) & (zp ? 0xFFu : 0xFFFFu);
                     return u_mem_lane<D0, D1>(u, e, s, base, ea, pcn, cyc);
                 }
+            }
+            constexpr uint32_t kind = (D0 >> D_KIND_SHIFT) & 7u;
+            constexpr bool poll = ixsel == 0u && !zp && (D0 & D_RD) != 0u && (D0 & D_WR) == 0u &&
+                                  (kind == K_BIT || (kind == K_LOGIC && (D0 & D_M_LD) != 0u));
+            if constexpr (poll) {
+                // INTIM / INSTAT, the addresses u_io_read answers from the timer
+                if (__builtin_expect((base & 0x1286u) == 0x284u, 0)) u_poll<kind == K_BIT>(u, e, base & 0x1FFFu, pcn);
             }
             const uint32_t ea = (base + iu) & (zp ? 0xFFu : 0xFFFFu);
             return u_mem_uni<D0, D1>(u, e, s, base, ea, pcn, cyc);
@@ -591,18 +671,7 @@ __device__ __forceinline__ uint32_t u_run(UCtl &u, Emu &e, uint32_t &st) {
 #undef FA_UCASE
             default: return U_EXIT;
             }
-            u.cycles += cyc;
-            u.cpu += cyc;
-            u.instr += 1u;
-            u_tia_tick(u, e, cyc);
-            // Pia::tick_n's fast branch inline, the rest out of line
-            if (__builtin_expect(u.tlim > cyc, 1)) {
-                u.tps -= cyc;
-                u.tlim -= cyc;
-            } else {
-                timer_tick_slow(u, cyc);
-                u.tlim = u_tlim(u);
-            }
+            u_step(u, e, cyc);
             const uint32_t o = ROM0 + (u.bank << 12) + (u.pc & 0xFFFu);
             f0 = ld8(o); f1 = ld8(o + 1u); f2 = ld8(o + 2u);
         }

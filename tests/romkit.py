@@ -130,3 +130,59 @@ def io_rom():
                  0x4C, 0x00, 0xF1])                                # JMP $F100
     rom[0xFFC], rom[0xFFD] = 0x00, 0xF0
     return bytes(rom)
+
+
+def poll_rom(end="frame"):
+    """Fixed 4 KB ROM of RIOT timer polls: a load of INTIM / INSTAT followed by
+    a branch back to it, the way a kernel waits out VBLANK and overscan.  One
+    frame runs: a poll inside loop A (VSYNC held); a per-env delay (SWCHA's
+    low bits) and a poll at per-env timer phases; then, with the lanes
+    realigned by WSYNC and the timer rewritten, BIT INSTAT / BPL, LDA INTIM /
+    BNE, LDX INTIM / BPL, BIT INTIM / BVC, LDY (an INTIM mirror) / BNE,
+    LDA INTIM / BMI after an underflow; polls whose branch flag is per env
+    (BIT / BNE with a per-env A; LDA / BCC) and one whose branch crosses a
+    page.  `end` = "endless_b" or "endless_a" adds a BIT INSTAT / BMI loop
+    that never leaves, after VSYNC drops (the frame ends at loop B's cap) or
+    while VSYNC is held (loop A's cap, status VSYNC held)."""
+    rom = bytearray([0xEA] * 4096)
+
+    def put(addr, prog):
+        rom[addr - 0xF000: addr - 0xF000 + len(prog)] = bytes(prog)
+
+    endless = [0xA9, 0x01, 0x8D, 0x94, 0x02,                      # LDA #1; STA TIM1T
+               0x2C, 0x85, 0x02, 0x10, 0xFB,                      # BIT INSTAT; BPL *-3
+               0x2C, 0x85, 0x02, 0x30, 0xFB]                      # BIT INSTAT; BMI *-3 (flag stays set)
+    put(0xF000, [0x78, 0xD8, 0xA2, 0xFF, 0x9A, 0x4C, 0x00, 0xF1])  # SEI; CLD; LDX #$FF; TXS; JMP $F100
+    put(0xF100, [0xA9, 0x02, 0x85, 0x00,                          # VSYNC on
+                 0xA9, 0x02, 0x8D, 0x96, 0x02,                    # LDA #2; STA TIM64T
+                 0x2C, 0x85, 0x02, 0x10, 0xFB]                    # BIT INSTAT; BPL *-3 (loop A)
+                + (endless if end == "endless_a" else []) +
+                [0x85, 0x02, 0xA9, 0x00, 0x85, 0x00,              # STA WSYNC; VSYNC off
+                 0xE6, 0x80,                                      # INC $80
+                 0xAD, 0x80, 0x02, 0x29, 0x03, 0xAA,              # LDA SWCHA; AND #3; TAX
+                 0xCA, 0x10, 0xFD,                                # DEX; BPL *-1: per-env delay
+                 0xA9, 0x14, 0x8D, 0x96, 0x02,                    # LDA #20; STA TIM64T
+                 0x2C, 0x85, 0x02, 0x10, 0xFB,                    # BIT INSTAT; BPL (per-env phases)
+                 0x85, 0x02,                                      # STA WSYNC
+                 0xA9, 0x1E, 0x8D, 0x95, 0x02,                    # LDA #30; STA TIM8T
+                 0xAD, 0x84, 0x02, 0xD0, 0xFB,                    # LDA INTIM; BNE
+                 0xA9, 0x05, 0x8D, 0x94, 0x02,                    # LDA #5; STA TIM1T
+                 0xAE, 0x84, 0x02, 0x10, 0xFB,                    # LDX INTIM; BPL
+                 0xA9, 0x03, 0x8D, 0x97, 0x02,                    # LDA #3; STA T1024T
+                 0x2C, 0x84, 0x02, 0x50, 0xFB,                    # BIT INTIM; BVC
+                 0xA9, 0x10, 0x8D, 0x95, 0x02,                    # LDA #16; STA TIM8T
+                 0xAC, 0x8C, 0x02, 0xD0, 0xFB,                    # LDY $028C (INTIM); BNE
+                 0xA9, 0x01, 0x8D, 0x94, 0x02,                    # LDA #1; STA TIM1T
+                 0x2C, 0x85, 0x02, 0x10, 0xFB,                    # BIT INSTAT; BPL (underflow)
+                 0xAD, 0x84, 0x02, 0x30, 0xFB,                    # LDA INTIM; BMI
+                 0xA9, 0x20, 0x8D, 0x95, 0x02,                    # LDA #32; STA TIM8T
+                 0xAD, 0x80, 0x02, 0x09, 0x01,                    # LDA SWCHA; ORA #1
+                 0x2C, 0x84, 0x02, 0xD0, 0xFB,                    # BIT INTIM; BNE (Z per env)
+                 0x38, 0xAD, 0x84, 0x02, 0x90, 0xFB,              # SEC; LDA INTIM; BCC (C per env)
+                 0xA9, 0x04, 0x8D, 0x95, 0x02,                    # LDA #4; STA TIM8T
+                 0x4C, 0xFD, 0xF2])                               # JMP $F2FD
+    put(0xF2FD, [0x2C, 0x85, 0x02, 0x10, 0xFB]                    # BIT INSTAT; BPL across $F300
+                + (endless if end == "endless_b" else []) +
+                [0x4C, 0x00, 0xF1])                               # JMP $F100
+    rom[0xFFC], rom[0xFFD] = 0x00, 0xF0
+    return bytes(rom)

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from gpu_util import assert_same
-from romkit import branch_hotspot_rom, io_rom, jsr_rom, stack_rom
+from romkit import branch_hotspot_rom, io_rom, jsr_rom, poll_rom, stack_rom
 
 pytestmark = pytest.mark.gpu
 
@@ -415,3 +415,31 @@ def test_tia_and_riot_registers_at_per_env_addresses(fa, orc, W, sorted_):
         ref.run(a[None, :])
         assert np.array_equal(obs, ref.obs()), f"obs frame {f}"
         assert_same(gpu, ref, f"frame {f}")
+
+
+@pytest.mark.parametrize("sorted_", [False, True])
+@pytest.mark.parametrize("end,acts", [("frame", "varied"), ("frame", "same"), ("endless_b", "varied"),
+                                      ("endless_a", "varied")])
+def test_timer_polls(fa, orc, W, end, acts, sorted_):
+    """RIOT timer polls (a load of INTIM / INSTAT and a branch back to it, on
+    every flag and load kind, INTIM mirrors, per-env flags, a page-crossing
+    branch) in the wave-uniform mode, with lanes realigned and their frame
+    cycle counts spread, at per-env timer phases on the per-lane path, and in
+    polls that run into loop B's and loop A's caps: every frame equal to the
+    oracle."""
+    rom = poll_rom(end)
+    n = 256
+    gpu, ref = _pair(fa, orc, rom, n, sorted_)
+    stream = W.ActionStream(0x5011, n)
+    for f in range(2 if end == "endless_a" else 4):
+        a = stream.next()
+        if acts == "same":
+            a = np.full(n, a[0], np.uint8)
+        gpu.step(a)
+        ref.run(a[None, :])
+        assert_same(gpu, ref, f"frame {f}")
+    c = ref.counters()
+    if end == "endless_b":
+        assert (c[:, 1] > 100000).all()
+    if end == "endless_a":
+        assert (c[:, 3] == 2).all()
