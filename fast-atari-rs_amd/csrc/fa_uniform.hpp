@@ -33,7 +33,7 @@ struct UCtl {
     uint32_t tlim;           // timer fast-path budget (u_tlim)
     uint32_t cycles, in_a;   // run_frame's cycle count (with cspread: a bound, see u_enter) and loop-A flag
     uint32_t c0, cspread;    // cycles at entry; non-zero when the lanes' counts differ
-    uint32_t cpu, instr;     // CPU cycles and instructions executed since entry
+    uint32_t skip, instr;    // WSYNC-skip cycles and instructions run since entry (the CPU's cycles: cycles - c0 - skip)
 };
 
 enum : uint32_t { U_DONE = 0, U_EXIT = 1 };
@@ -107,7 +107,7 @@ __device__ __forceinline__ bool u_enter(const Emu &e, uint32_t cycles, bool in_a
     u.vsync = (s4 >> 8) & 0xFFu;
     u.in_a = (s4 >> 16) & 1u;
     u.wsync |= u_loop_bit(u);
-    u.cpu = 0;
+    u.skip = 0;
     u.instr = 0;
     return true;
 }
@@ -119,7 +119,7 @@ __device__ __forceinline__ void u_leave(const UCtl &u, Emu &e, uint32_t &cycles,
     e.tval = u.tval; e.tps = u.tps; e.tsel = u.tsel; e.tsh = u.tsh; e.tuf = u.tuf;
     cycles += u.cycles - u.c0;
     in_a = u.in_a != 0u;
-    cpu_cyc += u.cpu;
+    cpu_cyc += u.cycles - u.c0 - u.skip;
     instr += u.instr;
 }
 
@@ -155,7 +155,6 @@ __device__ __forceinline__ uint32_t u_tlim(const UCtl &u) { return (u.tval == 0u
 // the TIA clock, and Pia::tick_n (its fast branch inline, the rest out of line).
 __device__ __forceinline__ void u_step(UCtl &u, Emu &e, uint32_t cyc) {
     u.cycles += cyc;
-    u.cpu += cyc;
     u.instr += 1u;
     u_tia_tick(u, e, cyc);
     if (__builtin_expect(u.tlim > cyc, 1)) {
@@ -440,7 +439,6 @@ __device__ __forceinline__ void u_poll(UCtl &u, Emu &e, uint32_t a13, uint32_t p
         k = __builtin_elementwise_min(k, (kLimit - u.cycles) / kIter);
         if (u.tlim != 0u && k != 0u) {
             u.cycles += kIter * k;
-            u.cpu += kIter * k;
             u.instr += 2u * k;
             u.clock += 3u * kIter * k;
             u.tps -= kIter * k;
@@ -622,7 +620,7 @@ __device__ __forceinline__ uint32_t u_run(UCtl &u, Emu &e, uint32_t &st) {
     u.tsh = wave_first(u.tsh); u.tuf = wave_first(u.tuf);
     u.tlim = u_tlim(u);
     u.cycles = wave_first(u.cycles); u.in_a = wave_first(u.in_a); u.cspread = wave_first(u.cspread);
-    u.cpu = 0;
+    u.skip = 0;
     u.instr = 0;
     // opcode and operand bytes at PC, loaded one iteration ahead so that the
     // LDS latency overlaps the previous instruction's timing work (the address
@@ -650,6 +648,7 @@ __device__ __forceinline__ uint32_t u_run(UCtl &u, Emu &e, uint32_t &st) {
                 timer_tick(u, s);
                 u.tlim = u_tlim(u);
                 u.cycles += s;
+                u.skip += s;
                 goto caps;
             }
         }
