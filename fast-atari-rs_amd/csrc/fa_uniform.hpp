@@ -310,17 +310,52 @@ __device__ __forceinline__ uint32_t u_alu(Emu &e, uint32_t s, uint32_t v) {
     return r;
 }
 
-// Memory operand at a per-lane address: every lane must access RAM, no
-// discarded fix-up read may have a side effect, and the page crossing (a cycle)
-// must agree across lanes.
+// A data read at per-lane addresses none of which has a side effect (no
+// hotspot, no RIOT timer register), as the per-lane stage makes it: one load
+// from the ROM window in the current bank, a RAM mirror, the TIA register's
+// image byte (fixed by tia_fix) or a zero byte (the flat array's padding);
+// the RIOT ports (Pia::read, src/pia.rs:182-200) from the lane's port bytes.
+// Per-lane choices are selects (see u_alu's section note).
+__device__ __forceinline__ uint32_t u_read_lanes(const UCtl &u, const Emu &e, uint32_t a13) {
+    const bool rom = (a13 & 0x1000u) != 0u;
+    const bool ram = (a13 & 0x1E80u) == 0x0080u;
+    const bool tia = (a13 & 0x1080u) == 0u;
+    const uint32_t r4 = a13 & 0x0Fu;
+    uint32_t ad = pick(tia, e.img + S_CX + Emu::tia_lut(r4), ZERO_BYTE);
+    ad = pick(ram, e.ram_off(a13), ad);
+    ad = pick(rom, ROM0 + (u.bank << 12) + (a13 & 0xFFFu), ad);
+    uint32_t m = ld8(ad);
+    m = pick(tia, e.tia_fix(r4, m), m);
+    const bool riot = (a13 & 0x1280u) == 0x0280u;
+    if (__builtin_expect(__ballot(riot) != 0, 0)) {
+        // port A at S_PA_OUT (output, DDR, input), port B three bytes on
+        const uint32_t pb = e.img + S_PA_OUT + 3u * ((a13 >> 1) & 1u);
+        const uint32_t ddr = ld8(pb + 1u);
+        const uint32_t port = (ld8(pb + 2u) & ~ddr & 0xFFu) | (ld8(pb) & ddr);
+        m = pick(riot, pick((a13 & 1u) != 0u, ddr, port), m);
+    }
+    return m;
+}
+
+// Memory operand at a per-lane address: every lane must access RAM (a plain
+// read: any region whose read has no side effect, u_read_lanes), no
+// discarded fix-up read may have a side effect, and the page crossing (a
+// cycle) must agree across lanes.
 template <uint32_t D0, uint32_t D1>
 __device__ __forceinline__ bool u_mem_lane(UCtl &u, Emu &e, uint32_t s, uint32_t base, uint32_t ea, uint32_t pcn,
                                            uint32_t &cyc) {
     constexpr bool rd = (D0 & D_RD) != 0u, wr = (D0 & D_WR) != 0u;
     constexpr bool dalways = (D0 & D_DUMMY_ALWAYS) != 0u, dcross = (D0 & D_DUMMY_CROSS) != 0u;
+    constexpr bool any = rd && !wr;
     const uint32_t a13 = ea & 0x1FFFu;
     const bool cross = ((base ^ ea) & 0xFF00u) != 0u;
-    bool bad = rd ? (a13 & 0x1E80u) != 0x0080u : (a13 & 0x1280u) != 0x0080u;
+    const bool ram = rd ? (a13 & 0x1E80u) == 0x0080u : (a13 & 0x1280u) == 0x0080u;
+    bool bad = !ram;
+    if constexpr (any) {
+        // a hotspot switches the bank; INTIM clears the underflow flag (and the
+        // timer registers are wave-uniform state here)
+        bad = ((((a13 & 0x1000u) != 0u) & (((a13 & 0xFFFu) - e.hot_lo) < e.hot_n))) | ((a13 & 0x1284u) == 0x0284u);
+    }
     if constexpr (dalways || dcross) {
         // a discarded fix-up read at a hotspot or INTIM has a side effect
         const uint32_t fa = ((base & 0xFF00u) | (ea & 0xFFu)) & 0x1FFFu;
@@ -335,7 +370,11 @@ __device__ __forceinline__ bool u_mem_lane(UCtl &u, Emu &e, uint32_t s, uint32_t
         cyc += cm != 0 ? 1u : 0u;
     }
     const uint32_t off = e.ram_off(a13);
-    const uint32_t v = rd ? ld8(off) : s;
+    uint32_t v = s;
+    if constexpr (rd) {
+        if (!any || __ballot(!ram) == 0) v = ld8(off);
+        else v = u_read_lanes(u, e, a13);
+    }
     const uint32_t r = u_alu<D0, D1>(e, s, v);
     if constexpr (wr) st8(off, r);
     u.pc = pcn;

@@ -186,3 +186,48 @@ def poll_rom(end="frame"):
                 [0x4C, 0x00, 0xF1])                               # JMP $F100
     rom[0xFFC], rom[0xFFD] = 0x00, 0xF0
     return bytes(rom)
+
+
+def lane_read_rom():
+    """F8 ROM (both banks the same code, different data) whose reads use a
+    per-env index while the control flow stays the same on every env: X is a
+    per-env walk ($90 + SWCHA, written back each frame) and Y = X & $3F.  Per
+    frame, after selecting bank 1 at a fixed address: LDA $F200,X (a ROM
+    table of the current bank), LDA $0400,X (TIA mirrors for X < $80, the
+    flat array's padding above), LDA $0280,X & 3 ports through TAX of X & 3
+    (the RIOT ports), LDA ($92),Y with the pointer in ROM ($F1C0), LDA
+    $F2F0,X (a page crossing on some envs only), then LDA $FFC0,X, which
+    reaches the hotspots $FFF8 / $FFF9 (bank switch) on some envs and wraps to
+    $0000+ on others.  Each value is folded into its own RAM byte."""
+    rom = bytearray([0xEA] * 8192)
+    for b in range(2):
+        o = b * 4096
+
+        def put(addr, prog):
+            rom[o + addr - 0xF000: o + addr - 0xF000 + len(prog)] = bytes(prog)
+
+        put(0xF000, [0x78, 0xD8, 0xA2, 0xFF, 0x9A,                     # SEI; CLD; LDX #$FF; TXS
+                     0xA9, 0xC0, 0x85, 0x92, 0xA9, 0xF1, 0x85, 0x93,   # ($92) = $F1C0
+                     0x4C, 0x00, 0xF1])                                # JMP $F100
+        put(0xF100, [0xA9, 0x02, 0x85, 0x00, 0x85, 0x02,               # VSYNC on; STA WSYNC
+                     0xA9, 0x00, 0x85, 0x00,                           # VSYNC off
+                     0xAD, 0xF9, 0xFF,                                 # LDA $FFF9: bank 1
+                     0xAD, 0x80, 0x02, 0x45, 0x90, 0x69, 0x1D, 0x85, 0x90,   # LDA SWCHA; EOR $90; ADC #$1D; STA $90
+                     0xAA,                                             # TAX
+                     0xBD, 0x00, 0xF2, 0x45, 0x81, 0x85, 0x81,         # LDA $F200,X; EOR $81; STA $81
+                     0xBD, 0x00, 0x04, 0x45, 0x82, 0x85, 0x82,         # LDA $0400,X; EOR $82; STA $82
+                     0x8A, 0x29, 0x03, 0xA8,                           # TXA; AND #3; TAY
+                     0xB9, 0x80, 0x02, 0x45, 0x86, 0x85, 0x86,         # LDA $0280,Y; EOR $86; STA $86
+                     0x8A, 0x29, 0x3F, 0xA8,                           # TXA; AND #$3F; TAY
+                     0xB1, 0x92, 0x45, 0x83, 0x85, 0x83,               # LDA ($92),Y; EOR $83; STA $83
+                     0xBD, 0xF0, 0xF2, 0x45, 0x85, 0x85, 0x85,         # LDA $F2F0,X; EOR $85; STA $85
+                     0x85, 0x02,                                       # STA WSYNC
+                     0xBD, 0xC0, 0xFF, 0x45, 0x84, 0x85, 0x84,         # LDA $FFC0,X; EOR $84; STA $84
+                     0x4C, 0x00, 0xF1])                                # JMP $F100
+        for i in range(256):                                           # per-bank data
+            rom[o + 0x200 + i] = (i * 7 + 0x35 * (b + 1)) & 0xFF
+            rom[o + 0x300 + i] = (i * 13 + 0x11 * (b + 3)) & 0xFF
+        for i in range(64):
+            rom[o + 0x1C0 + i] = (i * 5 + 0x60 * (b + 1)) & 0xFF
+        rom[o + 0xFFC: o + 0x1000] = bytes([0x00, 0xF0, 0x00, 0xF0])
+    return bytes(rom)

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from gpu_util import assert_same
-from romkit import branch_hotspot_rom, io_rom, jsr_rom, poll_rom, stack_rom
+from romkit import branch_hotspot_rom, io_rom, jsr_rom, lane_read_rom, poll_rom, stack_rom
 
 pytestmark = pytest.mark.gpu
 
@@ -443,3 +443,24 @@ def test_timer_polls(fa, orc, W, end, acts, sorted_):
         assert (c[:, 1] > 100000).all()
     if end == "endless_a":
         assert (c[:, 3] == 2).all()
+
+
+@pytest.mark.parametrize("sorted_", [False, True])
+@pytest.mark.parametrize("acts", ["varied", "same"])
+def test_per_env_reads_outside_ram(fa, orc, W, acts, sorted_):
+    """Reads at per-env addresses while the control flow is the same on every
+    env: a ROM table of the current bank, TIA mirrors and the flat array's
+    padding, the RIOT ports, (zp),Y with the pointer in ROM, a page crossing
+    on some envs only, and hotspot reads (bank switch) on some envs only;
+    every frame equal to the oracle."""
+    rom = lane_read_rom()
+    n = 256
+    gpu, ref = _pair(fa, orc, rom, n, sorted_)
+    stream = W.ActionStream(0x7A11, n)
+    for f in range(8):
+        a = stream.next()
+        if acts == "same":
+            a = np.full(n, a[0], np.uint8)
+        gpu.step(a)
+        ref.run(a[None, :])
+        assert_same(gpu, ref, f"frame {f}")
