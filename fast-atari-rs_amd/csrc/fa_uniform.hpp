@@ -13,7 +13,8 @@
 // An instruction whose effect on the control state would differ between lanes
 // -- a branch taken by some lanes only, a page crossing on some lanes only, TXS
 // / RTS / RTI with per-lane values, a VSYNC or timer write of per-lane values, a
-// per-lane address outside RAM -- is declined before anything changes: the
+// write at a per-lane address outside RAM, a read at per-lane addresses that
+// reach a hotspot or a RIOT timer register -- is declined before anything changes: the
 // frame loop writes the scalar state back to every lane and runs that
 // instruction on the per-lane path, and re-enters this mode once the lanes agree
 // again.  Behaviour is the reference's CPU emulator throughout
