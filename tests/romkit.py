@@ -231,3 +231,39 @@ def lane_read_rom():
             rom[o + 0x1C0 + i] = (i * 5 + 0x60 * (b + 1)) & 0xFF
         rom[o + 0xFFC: o + 0x1000] = bytes([0x00, 0xF0, 0x00, 0xF0])
     return bytes(rom)
+
+
+def debt_rom():
+    """Fixed 4 KB ROM of indexed reads whose page crossing differs between
+    envs (LDA $F2F0,X with a per-env X: a cycle more on the envs with X >=
+    $10) while the control flow is the same on every env, each followed by
+    something that sees the cycle or not: two crossings then WSYNC; an INTIM
+    read then WSYNC; a RESP0 write; a timer write; 40 INC (a line end before
+    the next WSYNC); a timer poll (BIT INSTAT / BPL); and a VSYNC write at the
+    next frame's start."""
+    rom = bytearray([0xEA] * 4096)
+
+    def put(addr, prog):
+        rom[addr - 0xF000: addr - 0xF000 + len(prog)] = bytes(prog)
+
+    cross = [0xBD, 0xF0, 0xF2]                                         # LDA $F2F0,X
+    wsync = [0x85, 0x02]
+    prog = ([0xA9, 0x02, 0x85, 0x00, 0x85, 0x02, 0xA9, 0x00, 0x85, 0x00,   # VSYNC on; WSYNC; VSYNC off
+             0xAD, 0x80, 0x02, 0x45, 0x90, 0x69, 0x1D, 0x85, 0x90, 0xAA,   # X = per-env walk
+             0xA9, 0xC8, 0x8D, 0x96, 0x02]                             # LDA #200; STA TIM64T
+            + cross + [0x85, 0x81] + cross + [0x45, 0x81, 0x85, 0x81] + wsync
+            + cross + [0xAD, 0x84, 0x02, 0x85, 0x82] + wsync           # LDA INTIM; STA $82
+            + cross + [0x85, 0x10] + wsync                             # STA RESP0
+            + cross + [0xA9, 0x64, 0x8D, 0x96, 0x02] + wsync           # LDA #100; STA TIM64T
+            + cross + [0xE6, 0x83] * 40 + wsync                        # 40 x INC $83
+            + [0xA9, 0x03, 0x8D, 0x95, 0x02] + cross                   # LDA #3; STA TIM8T; LDA $F2F0,X
+            + [0x2C, 0x85, 0x02, 0x10, 0xFB] + wsync                   # BIT INSTAT; BPL *-3
+            + cross + [0x85, 0x84]                                     # STA $84
+            + [0x4C, 0x00, 0xF1])                                      # JMP $F100 (VSYNC next)
+    assert len(prog) < 0x1F0
+    put(0xF000, [0x78, 0xD8, 0xA2, 0xFF, 0x9A, 0x4C, 0x00, 0xF1])      # SEI; CLD; LDX #$FF; TXS; JMP $F100
+    put(0xF100, prog)
+    for i in range(0x100):
+        rom[0x2F0 + i] = (i * 11 + 3) & 0xFF
+    rom[0xFFC], rom[0xFFD] = 0x00, 0xF0
+    return bytes(rom)

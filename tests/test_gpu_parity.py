@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from gpu_util import assert_same
-from romkit import branch_hotspot_rom, io_rom, jsr_rom, lane_read_rom, poll_rom, stack_rom
+from romkit import branch_hotspot_rom, debt_rom, io_rom, jsr_rom, lane_read_rom, poll_rom, stack_rom
 
 pytestmark = pytest.mark.gpu
 
@@ -457,6 +457,27 @@ def test_per_env_reads_outside_ram(fa, orc, W, acts, sorted_):
     n = 256
     gpu, ref = _pair(fa, orc, rom, n, sorted_)
     stream = W.ActionStream(0x7A11, n)
+    for f in range(8):
+        a = stream.next()
+        if acts == "same":
+            a = np.full(n, a[0], np.uint8)
+        gpu.step(a)
+        ref.run(a[None, :])
+        assert_same(gpu, ref, f"frame {f}")
+
+
+@pytest.mark.parametrize("sorted_", [False, True])
+@pytest.mark.parametrize("acts", ["varied", "same"])
+def test_page_crossings_on_some_envs(fa, orc, W, acts, sorted_):
+    """Indexed reads that cross a page on some envs only (a cycle more there)
+    while the control flow stays the same, followed by WSYNC, a second
+    crossing, an INTIM read, RESP0 and timer writes, a line end before the
+    next WSYNC, a timer poll and a VSYNC write: every frame equal to the
+    oracle."""
+    rom = debt_rom()
+    n = 256
+    gpu, ref = _pair(fa, orc, rom, n, sorted_)
+    stream = W.ActionStream(0x0DE8, n)
     for f in range(8):
         a = stream.next()
         if acts == "same":
